@@ -1,0 +1,173 @@
+"""Generate the Rys quadrature tables used by the HIP kernels and host code.
+
+Rys quadrature replaces the Boys-function sum of an electron-repulsion (or
+nuclear-attraction) integral by an n-point Gauss rule for the weight
+exp(-T t^2) on t in [0, 1], in the variable u = t^2:
+
+    int_0^1 exp(-T t^2) g(t^2) dt = sum_i w_i(T) g(u_i(T))    (deg g <= 2n-1)
+
+The moments of that weight are the Boys functions F_k(T) = int_0^1 t^2k
+exp(-T t^2) dt (the function the reference evaluates in
+src/special_functions.f90:189-248), so sum_i w_i = F_0(T).
+
+Method (all published, nothing tabulated by hand):
+  * moments F_0..F_{2n-1}(T) from the confluent hypergeometric form
+    F_k(T) = 1F1(k+1/2; k+3/2; -T) / (2k+1), in 60-digit arithmetic (mpmath);
+  * recurrence coefficients of the monic orthogonal polynomials by Gautschi's
+    (ordinary) Chebyshev algorithm - ill-conditioned in double precision, but
+    harmless at 60 digits;
+  * nodes and weights from the Jacobi matrix (Golub-Welsch);
+  * each u_i(T), w_i(T) is fitted on [0, TMAX) by piecewise Chebyshev
+    interpolation (NINT intervals of width H, degree DEG);
+  * for T >= TMAX the weight exp(-T t^2) is negligible at t = 1 and the rule
+    becomes half-range Gauss-Hermite: u_i = x_i^2 / T, w_i = W_i / sqrt(T)
+    with x_i > 0 the positive nodes of the 2n-point Gauss-Hermite rule and W_i
+    its weights.
+
+Output: rys_tables.inc (C/HIP source, doubles), layout
+
+    RYS_CHEB[n][interval][coef k][2*i + {0: u, 1: w}]   (flattened, see offsets)
+
+so that one lane evaluating root i of one interval reads 2*DEG+2 doubles
+that sit in one contiguous stretch per coefficient row.
+
+Run:  python hfsolver_amd/csrc/gen_rys_tables.py   (about a minute, 8 cores)
+"""
+import os
+import sys
+from multiprocessing import Pool
+
+import mpmath as mp
+
+NMAX = 5          # enough for (dd|dd) energies and their first derivatives (L <= 9)
+H = 1.0           # interval width in T
+NINT = 60         # intervals on [0, TMAX); exp(-T) T^(m-1/2)/Gamma(m+1/2) < 1e-16 at T = 60 for m <= 9
+TMAX = H * NINT
+DEG = 13          # Chebyshev degree per interval
+DPS = 60
+
+mp.mp.dps = DPS
+
+
+def boys_moments(T, kmax):
+    T = mp.mpf(T)
+    return [mp.hyp1f1(k + mp.mpf(1) / 2, k + mp.mpf(3) / 2, -T) / (2 * k + 1)
+            for k in range(kmax + 1)]
+
+
+def rys_exact(n, T):
+    """Nodes u_i and weights w_i (ascending u) of the n-point Rys rule at T."""
+    mu = boys_moments(T, 2 * n - 1)
+    # Gautschi's Chebyshev algorithm on ordinary moments.
+    alpha = [mp.mpf(0)] * n
+    beta = [mp.mpf(0)] * n
+    sig_prev = [mp.mpf(0)] * (2 * n)
+    sig = list(mu)
+    alpha[0] = mu[1] / mu[0]
+    beta[0] = mu[0]
+    for k in range(1, n):
+        sig_new = [mp.mpf(0)] * (2 * n)
+        for l in range(k, 2 * n - k):
+            sig_new[l] = sig[l + 1] - alpha[k - 1] * sig[l] - beta[k - 1] * sig_prev[l]
+        alpha[k] = sig_new[k + 1] / sig_new[k] - sig[k] / sig[k - 1]
+        beta[k] = sig_new[k] / sig[k - 1]
+        sig_prev, sig = sig, sig_new
+    J = mp.matrix(n, n)
+    for i in range(n):
+        J[i, i] = alpha[i]
+        if i + 1 < n:
+            J[i, i + 1] = J[i + 1, i] = mp.sqrt(beta[i + 1])
+    E, Q = mp.eigsy(J)
+    pairs = sorted((E[i], mu[0] * Q[0, i] ** 2) for i in range(n))
+    return [p[0] for p in pairs], [p[1] for p in pairs]
+
+
+def hermite_half(n):
+    """Positive nodes x_i^2 and weights of the 2n-point Gauss-Hermite rule."""
+    m = 2 * n
+    J = mp.matrix(m, m)
+    for i in range(m - 1):
+        J[i, i + 1] = J[i + 1, i] = mp.sqrt(mp.mpf(i + 1) / 2)
+    E, Q = mp.eigsy(J)
+    out = sorted((E[i] ** 2, mp.sqrt(mp.pi) * Q[0, i] ** 2) for i in range(m) if E[i] > 0)
+    return [o[0] for o in out], [o[1] for o in out]
+
+
+def fit_interval(args):
+    n, it = args
+    mp.mp.dps = DPS
+    a = mp.mpf(it) * H
+    vals = []  # vals[j] = (u list, w list) at node j
+    xs = []
+    for j in range(DEG + 1):
+        x = mp.cos(mp.pi * (j + mp.mpf(1) / 2) / (DEG + 1))
+        xs.append(x)
+        T = a + H * (x + 1) / 2
+        vals.append(rys_exact(n, T))
+    coefs = []  # coefs[k][2*i + s]
+    for k in range(DEG + 1):
+        row = []
+        for i in range(n):
+            for s in range(2):
+                acc = mp.mpf(0)
+                for j in range(DEG + 1):
+                    acc += vals[j][s][i] * mp.cos(mp.pi * k * (j + mp.mpf(1) / 2) / (DEG + 1))
+                acc *= mp.mpf(2) / (DEG + 1)
+                if k == 0:
+                    acc /= 2
+                row.append(float(acc))
+        coefs.append(row)
+    return n, it, coefs
+
+
+def main():
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rys_tables.inc")
+    jobs = [(n, it) for n in range(1, NMAX + 1) for it in range(NINT)]
+    with Pool(min(8, os.cpu_count() or 1)) as pool:
+        res = pool.map(fit_interval, jobs)
+    table = {(n, it): c for n, it, c in res}
+    lines = []
+    lines.append("// Generated by gen_rys_tables.py - do not edit.")
+    lines.append("// Piecewise-Chebyshev Rys nodes/weights on T in [0, %g), Gauss-Hermite beyond." % TMAX)
+    lines.append("#define RYS_NMAX %d" % NMAX)
+    lines.append("#define RYS_NINT %d" % NINT)
+    lines.append("#define RYS_DEG %d" % DEG)
+    lines.append("#define RYS_H %.17g" % H)
+    lines.append("#define RYS_TMAX %.17g" % TMAX)
+    # offset of n-block: sum_{m<n} NINT*(DEG+1)*2m doubles
+    offs = []
+    o = 0
+    for n in range(1, NMAX + 1):
+        offs.append(o)
+        o += NINT * (DEG + 1) * 2 * n
+    lines.append("#define RYS_CHEB_SIZE %d" % o)
+    lines.append("static const int RYS_CHEB_OFFSET[RYS_NMAX + 1] = {%s, %d};" %
+                 (", ".join(str(x) for x in offs), o))
+    lines.append("RYS_TABLE_QUALIFIER double RYS_CHEB[RYS_CHEB_SIZE] = {")
+    for n in range(1, NMAX + 1):
+        lines.append("// n = %d" % n)
+        for it in range(NINT):
+            for row in table[(n, it)]:
+                lines.append("  " + ", ".join("%.17e" % v for v in row) + ",")
+    lines.append("};")
+    # asymptotic Gauss-Hermite data: RYS_HERM_U[n-1][i], RYS_HERM_W[n-1][i]
+    hu, hw = [], []
+    for n in range(1, NMAX + 1):
+        r, w = hermite_half(n)
+        hu.append([float(x) for x in r] + [0.0] * (NMAX - n))
+        hw.append([float(x) for x in w] + [0.0] * (NMAX - n))
+    lines.append("RYS_TABLE_QUALIFIER double RYS_HERM_U[RYS_NMAX * RYS_NMAX] = {")
+    for row in hu:
+        lines.append("  " + ", ".join("%.17e" % v for v in row) + ",")
+    lines.append("};")
+    lines.append("RYS_TABLE_QUALIFIER double RYS_HERM_W[RYS_NMAX * RYS_NMAX] = {")
+    for row in hw:
+        lines.append("  " + ", ".join("%.17e" % v for v in row) + ",")
+    lines.append("};")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("wrote", out, "doubles:", o)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
