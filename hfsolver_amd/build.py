@@ -1,0 +1,59 @@
+"""Build the in-tree native library hfsolver_amd/libhfx.so (gfx950).
+
+    python -m hfsolver_amd.build            # library only
+    python -m hfsolver_amd.build --check    # also the host check harness of tests/
+
+The kernels (csrc/hfx_kernels.hip) are compiled for gfx950 only, with packed
+FP32 instructions disabled; the host runtime (csrc/hfx_host.cpp) with OpenMP.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libhfx.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "--offload-arch=" + ARCH]
+
+
+def run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError("build step failed: %s" % os.path.basename(cmd[-1]))
+    return r
+
+
+def build_library(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = [
+        DEVICE_FLAGS + ["-c", os.path.join(CSRC, "hfx_kernels.hip"), "-o", os.path.join(BUILD, "hfx_kernels.o")],
+        HOST_FLAGS + ["-c", os.path.join(CSRC, "hfx_host.cpp"), "-o", os.path.join(BUILD, "hfx_host.o")],
+    ]
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(lambda j: run([HIPCC] + j), jobs))
+    tmp = LIB + ".tmp"
+    run([HIPCC, "-shared", "-fopenmp", "--offload-arch=" + ARCH, "-o", tmp,
+         os.path.join(BUILD, "hfx_kernels.o"), os.path.join(BUILD, "hfx_host.o"),
+         "-Wl,-rpath,/opt/rocm/lib:/opt/rocm/llvm/lib"])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+def main(argv):
+    build_library()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -1,0 +1,325 @@
+// Contracted shell-quartet electron-repulsion integrals by Rys quadrature,
+// written once for host and device (template on the angular momenta and on
+// the arithmetic type R = double or float).
+//
+// Counterpart of hfsolver's contr_coulomb / coulomb_repulsion
+// (src/cints.c:72-149), which evaluate the same integrals one basis-function
+// quartet at a time by the THO closed form. Here one call evaluates a whole
+// (or a slice of a) Cartesian shell quartet (AB|CD), l_A >= l_B, l_C >= l_D:
+//
+//   (ab|cd) = sum_{prim pairs} 2 pi^{5/2} K_ab K_cd / (p q sqrt(p+q))
+//             * sum_{roots r} w_r Ix(r) Iy(r) Iz(r)
+//
+// with the 2-D integrals built by the Rys recurrences
+//   G(n+1,0) = C00 G(n,0) + n B10 G(n-1,0)
+//   G(n,m+1) = C00' G(n,m) + m B01 G(n,m-1) + n B00 G(n-1,m)
+//   C00 = PA - u rho/p PQ,  C00' = QC + u rho/q PQ,
+//   B10 = (1 - u rho/p)/(2p),  B01 = (1 - u rho/q)/(2q),  B00 = u/(2(p+q)),
+// followed by the horizontal transfers I(a,b+1) = I(a+1,b) + AB I(a,b)
+// (bra) and the same on the ket with CD.
+//
+// Splitting: for the large classes one thread evaluates only the slice of a
+// single a-component (SPA) and/or a single c-component (SPC); the component is
+// a run-time, wavefront-uniform index, selected with conditional moves so that
+// every array stays in registers.
+#pragma once
+
+#include "hfx_common.h"
+#include "rys.h"
+
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC>
+struct QClass {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int NAt = SPA ? 1 : NA;
+    static constexpr int NCt = SPC ? 1 : NC;
+    static constexpr int NOUT = NAt * NB * NCt * ND;
+    static constexpr int LAB = LA + LB, LCD = LC + LD;
+    static constexpr int NR = (LAB + LCD) / 2 + 1;
+    static constexpr int XA = SPA ? 1 : LA + 1;   // a-exponent extent kept after selection
+    static constexpr int XC = SPC ? 1 : LC + 1;
+};
+
+// Split policy of the engine: slice over a (then over c) while a thread would
+// hold more than 81 integrals.
+HFX_HD constexpr bool split_a(int la, int lb, int lc, int ld) {
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > 81;
+}
+HFX_HD constexpr bool split_c(int la, int lb, int lc, int ld) {
+    return split_a(la, lb, lc, ld) && ncart(lb) * ncart(lc) * ncart(ld) > 81;
+}
+HFX_HD constexpr int nsplit(int la, int lb, int lc, int ld) {
+    return (split_a(la, lb, lc, ld) ? ncart(la) : 1) * (split_c(la, lb, lc, ld) ? ncart(lc) : 1);
+}
+
+// One Cartesian direction of one Rys root: 2-D integrals, bra and ket
+// transfers, selection of the a / c exponent (ea, ec) when slicing.
+// out[ia][ib][ic][id] for ia < XA, ib <= LB, ic < XC, id <= LD.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void rys_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R g00, int ea, int ec,
+                               R (&out)[QClass<LA, LB, LC, LD, SPA, SPC>::XA][LB + 1]
+                                       [QClass<LA, LB, LC, LD, SPA, SPC>::XC][LD + 1]) {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    constexpr int LAB = Q::LAB, LCD = Q::LCD;
+    R G[LAB + 1][LCD + 1];
+    G[0][0] = g00;
+    if (LAB > 0) G[1][0] = c00 * g00;
+#pragma unroll
+    for (int n = 1; n < LAB; n++) G[n + 1][0] = c00 * G[n][0] + R(n) * b10 * G[n - 1][0];
+#pragma unroll
+    for (int m = 0; m < LCD; m++) {
+#pragma unroll
+        for (int n = 0; n <= LAB; n++) {
+            R v = d00 * G[n][m];
+            if (m > 0) v += R(m) * b01 * G[n][m - 1];
+            if (n > 0) v += R(n) * b00 * G[n - 1][m];
+            G[n][m + 1] = v;
+        }
+    }
+    // bra transfer: H[ia][ib][m], ia + ib <= LAB
+    R H[LAB + 1][LB + 1][LCD + 1];
+#pragma unroll
+    for (int ia = 0; ia <= LAB; ia++)
+#pragma unroll
+        for (int m = 0; m <= LCD; m++) H[ia][0][m] = G[ia][m];
+#pragma unroll
+    for (int ib = 1; ib <= LB; ib++)
+#pragma unroll
+        for (int ia = 0; ia <= LAB - ib; ia++)
+#pragma unroll
+            for (int m = 0; m <= LCD; m++) H[ia][ib][m] = H[ia + 1][ib - 1][m] + ab * H[ia][ib - 1][m];
+    // keep ia <= LA (or the selected ia = ea)
+    R HA[Q::XA][LB + 1][LCD + 1];
+#pragma unroll
+    for (int xa = 0; xa < Q::XA; xa++)
+#pragma unroll
+        for (int ib = 0; ib <= LB; ib++)
+#pragma unroll
+            for (int m = 0; m <= LCD; m++) {
+                if (SPA) {
+                    R v = H[0][ib][m];
+#pragma unroll
+                    for (int t = 1; t <= LA; t++) v = (ea == t) ? H[t][ib][m] : v;
+                    HA[xa][ib][m] = v;
+                } else {
+                    HA[xa][ib][m] = H[xa][ib][m];
+                }
+            }
+    // ket transfer: K[xa][ib][ic][id], ic + id <= LCD
+    R Kt[Q::XA][LB + 1][LCD + 1][LD + 1];
+#pragma unroll
+    for (int xa = 0; xa < Q::XA; xa++)
+#pragma unroll
+        for (int ib = 0; ib <= LB; ib++) {
+#pragma unroll
+            for (int ic = 0; ic <= LCD; ic++) Kt[xa][ib][ic][0] = HA[xa][ib][ic];
+#pragma unroll
+            for (int id = 1; id <= LD; id++)
+#pragma unroll
+                for (int ic = 0; ic <= LCD - id; ic++)
+                    Kt[xa][ib][ic][id] = Kt[xa][ib][ic + 1][id - 1] + cd * Kt[xa][ib][ic][id - 1];
+        }
+#pragma unroll
+    for (int xa = 0; xa < Q::XA; xa++)
+#pragma unroll
+        for (int ib = 0; ib <= LB; ib++)
+#pragma unroll
+            for (int xc = 0; xc < Q::XC; xc++)
+#pragma unroll
+                for (int id = 0; id <= LD; id++) {
+                    if (SPC) {
+                        R v = Kt[xa][ib][0][id];
+#pragma unroll
+                        for (int t = 1; t <= LC; t++) v = (ec == t) ? Kt[xa][ib][t][id] : v;
+                        out[xa][ib][xc][id] = v;
+                    } else {
+                        out[xa][ib][xc][id] = Kt[xa][ib][xc][id];
+                    }
+                }
+}
+
+// Contracted integrals of the shell quartet (bra | ket), slice (sa, sc):
+// eri[((a*NB + b)*NCt + c)*ND + d] for the thread's a- and c-components.
+// bra/ket: shell pairs with l_A = LA, l_B = LB, l_C = LC, l_D = LD.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, const PrimPair* __restrict__ prims,
+                                   int sa, int sc, const RysTables& tab,
+                                   R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT]) {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    constexpr int NR = Q::NR;
+#pragma unroll
+    for (int i = 0; i < Q::NOUT; i++) eri[i] = R(0);
+    // exponents of the selected a / c component (run-time, wavefront-uniform)
+    int sel_a[3] = {0, 0, 0}, sel_c[3] = {0, 0, 0};
+    if (SPA) {
+#pragma unroll
+        for (int i = 0; i < Q::NA; i++)
+            if (sa == i) {
+                sel_a[0] = cart_exp(LA, i, 0);
+                sel_a[1] = cart_exp(LA, i, 1);
+                sel_a[2] = cart_exp(LA, i, 2);
+            }
+    }
+    if (SPC) {
+#pragma unroll
+        for (int i = 0; i < Q::NC; i++)
+            if (sc == i) {
+                sel_c[0] = cart_exp(LC, i, 0);
+                sel_c[1] = cart_exp(LC, i, 1);
+                sel_c[2] = cart_exp(LC, i, 2);
+            }
+    }
+    const R abx = R(bra.ABx), aby = R(bra.ABy), abz = R(bra.ABz);
+    const R cdx = R(ket.ABx), cdy = R(ket.ABy), cdz = R(ket.ABz);
+    for (int ip = 0; ip < bra.np; ip++) {
+        const PrimPair P = prims[bra.p0 + ip];
+        for (int kp = 0; kp < ket.np; kp++) {
+            const PrimPair Qp = prims[ket.p0 + kp];
+            const double p = P.p, q = Qp.p, pq = p + q;
+            const double ipq = 1.0 / pq;
+            const double rho = p * q * ipq;
+            const double PQx = P.Px - Qp.Px, PQy = P.Py - Qp.Py, PQz = P.Pz - Qp.Pz;
+            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const double pref = HFX_TWO_PI_52 * P.K * Qp.K / (p * q * sqrt(pq));
+            double u[NR], w[NR];
+            rys_roots<NR>(T, u, w, tab);
+            const double ip = 1.0 / p, iq = 1.0 / q;
+            // the ket PrimPair stores Q - C in its PA fields
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const double ur = u[r];
+                const double t1 = ur * rho;
+                const R b00 = R(0.5 * ur * ipq);
+                const R b10 = R(0.5 * ip * (1.0 - t1 * ip));
+                const R b01 = R(0.5 * iq * (1.0 - t1 * iq));
+                const double fp = t1 * ip, fq = t1 * iq;
+                R Ix[Q::XA][LB + 1][Q::XC][LD + 1];
+                R Iy[Q::XA][LB + 1][Q::XC][LD + 1];
+                R Iz[Q::XA][LB + 1][Q::XC][LD + 1];
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAx - fp * PQx), R(Qp.PAx + fq * PQx), b10, b01, b00, abx,
+                                                     cdx, R(1), sel_a[0], sel_c[0], Ix);
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAy - fp * PQy), R(Qp.PAy + fq * PQy), b10, b01, b00, aby,
+                                                     cdy, R(1), sel_a[1], sel_c[1], Iy);
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAz - fp * PQz), R(Qp.PAz + fq * PQz), b10, b01, b00, abz,
+                                                     cdz, R(w[r] * pref), sel_a[2], sel_c[2], Iz);
+#pragma unroll
+                for (int a = 0; a < Q::NAt; a++) {
+                    const int ax = SPA ? 0 : cart_exp(LA, a, 0), ay = SPA ? 0 : cart_exp(LA, a, 1),
+                              az = SPA ? 0 : cart_exp(LA, a, 2);
+#pragma unroll
+                    for (int b = 0; b < Q::NB; b++) {
+                        const int bx = cart_exp(LB, b, 0), by = cart_exp(LB, b, 1), bz = cart_exp(LB, b, 2);
+#pragma unroll
+                        for (int c = 0; c < Q::NCt; c++) {
+                            const int cx = SPC ? 0 : cart_exp(LC, c, 0), cy = SPC ? 0 : cart_exp(LC, c, 1),
+                                      cz = SPC ? 0 : cart_exp(LC, c, 2);
+#pragma unroll
+                            for (int d = 0; d < Q::ND; d++) {
+                                const int dx = cart_exp(LD, d, 0), dy = cart_exp(LD, d, 1), dz = cart_exp(LD, d, 2);
+                                const R xy = Ix[ax][bx][cx][dx] * Iy[ay][by][cy][dy];
+                                R& e = eri[((a * Q::NB + b) * Q::NCt + c) * Q::ND + d];
+                                e = xy * Iz[az][bz][cz][dz] + e;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// J/K digestion of one slice of a quartet, accumulated in registers and
+// flushed through `add(ptr, value)` (atomics on the device, plain adds on the
+// host). Buffers J, K are n x n (row index = first function); the caller
+// symmetrises J = Jb + Jb^T, K = Kb + Kb^T afterwards. s = quartet degeneracy
+// factor (1/2 for A==B, for C==D and for bra pair == ket pair).
+//   Jb_ab += 2 s sum_cd (ab|cd) D_cd,  Jb_cd += 2 s sum_ab (ab|cd) D_ab,
+//   Kb_ac += s (ab|cd) D_bd,  Kb_ad += s (ab|cd) D_bc,
+//   Kb_bc += s (ab|cd) D_ad,  Kb_bd += s (ab|cd) D_ac.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R, typename Add>
+HFX_HD HFX_INLINE void digest_jk(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], int fa, int fb, int fc,
+                                 int fd, const double* __restrict__ D, int n, double s, double* J, double* K,
+                                 Add add) {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
+    R Dab[NAt][NB], Dcd[NCt][ND], Dbd[NB][ND], Dbc[NB][NCt], Dad[NAt][ND], Dac[NAt][NCt];
+#pragma unroll
+    for (int a = 0; a < NAt; a++) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) Dab[a][b] = R(D[(size_t)(fa + a) * n + fb + b]);
+#pragma unroll
+        for (int d = 0; d < ND; d++) Dad[a][d] = R(D[(size_t)(fa + a) * n + fd + d]);
+#pragma unroll
+        for (int c = 0; c < NCt; c++) Dac[a][c] = R(D[(size_t)(fa + a) * n + fc + c]);
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) Dbd[b][d] = R(D[(size_t)(fb + b) * n + fd + d]);
+#pragma unroll
+        for (int c = 0; c < NCt; c++) Dbc[b][c] = R(D[(size_t)(fb + b) * n + fc + c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NCt; c++)
+#pragma unroll
+        for (int d = 0; d < ND; d++) Dcd[c][d] = R(D[(size_t)(fc + c) * n + fd + d]);
+
+    R Jab[NAt][NB], Jcd[NCt][ND], Kac[NAt][NCt], Kad[NAt][ND], Kbc[NB][NCt], Kbd[NB][ND];
+#pragma unroll
+    for (int a = 0; a < NAt; a++) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) Jab[a][b] = R(0);
+#pragma unroll
+        for (int c = 0; c < NCt; c++) Kac[a][c] = R(0);
+#pragma unroll
+        for (int d = 0; d < ND; d++) Kad[a][d] = R(0);
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+#pragma unroll
+        for (int c = 0; c < NCt; c++) Kbc[b][c] = R(0);
+#pragma unroll
+        for (int d = 0; d < ND; d++) Kbd[b][d] = R(0);
+    }
+#pragma unroll
+    for (int c = 0; c < NCt; c++)
+#pragma unroll
+        for (int d = 0; d < ND; d++) Jcd[c][d] = R(0);
+
+#pragma unroll
+    for (int a = 0; a < NAt; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int c = 0; c < NCt; c++)
+#pragma unroll
+                for (int d = 0; d < ND; d++) {
+                    const R v = eri[((a * NB + b) * NCt + c) * ND + d];
+                    Jab[a][b] = v * Dcd[c][d] + Jab[a][b];
+                    Jcd[c][d] = v * Dab[a][b] + Jcd[c][d];
+                    Kac[a][c] = v * Dbd[b][d] + Kac[a][c];
+                    Kad[a][d] = v * Dbc[b][c] + Kad[a][d];
+                    Kbc[b][c] = v * Dad[a][d] + Kbc[b][c];
+                    Kbd[b][d] = v * Dac[a][c] + Kbd[b][d];
+                }
+    const double s2 = 2.0 * s;
+#pragma unroll
+    for (int a = 0; a < NAt; a++) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) add(&J[(size_t)(fa + a) * n + fb + b], s2 * double(Jab[a][b]));
+#pragma unroll
+        for (int c = 0; c < NCt; c++) add(&K[(size_t)(fa + a) * n + fc + c], s * double(Kac[a][c]));
+#pragma unroll
+        for (int d = 0; d < ND; d++) add(&K[(size_t)(fa + a) * n + fd + d], s * double(Kad[a][d]));
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+#pragma unroll
+        for (int c = 0; c < NCt; c++) add(&K[(size_t)(fb + b) * n + fc + c], s * double(Kbc[b][c]));
+#pragma unroll
+        for (int d = 0; d < ND; d++) add(&K[(size_t)(fb + b) * n + fd + d], s * double(Kbd[b][d]));
+    }
+#pragma unroll
+    for (int c = 0; c < NCt; c++)
+#pragma unroll
+        for (int d = 0; d < ND; d++) add(&J[(size_t)(fc + c) * n + fd + d], s2 * double(Jcd[c][d]));
+}

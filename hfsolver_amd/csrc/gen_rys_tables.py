@@ -24,7 +24,7 @@ Method (all published, nothing tabulated by hand):
     with x_i > 0 the positive nodes of the 2n-point Gauss-Hermite rule and W_i
     its weights.
 
-Output: rys_tables.inc (C/HIP source, doubles), layout
+Output: rys_params.h (sizes) and rys_tables.inc (C/HIP source, doubles), layout
 
     RYS_CHEB[n][interval][coef k][2*i + {0: u, 1: w}]   (flattened, see offsets)
 
@@ -126,23 +126,22 @@ def main():
     with Pool(min(8, os.cpu_count() or 1)) as pool:
         res = pool.map(fit_interval, jobs)
     table = {(n, it): c for n, it, c in res}
-    lines = []
-    lines.append("// Generated by gen_rys_tables.py - do not edit.")
-    lines.append("// Piecewise-Chebyshev Rys nodes/weights on T in [0, %g), Gauss-Hermite beyond." % TMAX)
-    lines.append("#define RYS_NMAX %d" % NMAX)
-    lines.append("#define RYS_NINT %d" % NINT)
-    lines.append("#define RYS_DEG %d" % DEG)
-    lines.append("#define RYS_H %.17g" % H)
-    lines.append("#define RYS_TMAX %.17g" % TMAX)
-    # offset of n-block: sum_{m<n} NINT*(DEG+1)*2m doubles
-    offs = []
     o = 0
     for n in range(1, NMAX + 1):
-        offs.append(o)
         o += NINT * (DEG + 1) * 2 * n
-    lines.append("#define RYS_CHEB_SIZE %d" % o)
-    lines.append("static const int RYS_CHEB_OFFSET[RYS_NMAX + 1] = {%s, %d};" %
-                 (", ".join(str(x) for x in offs), o))
+    hdr = ["// Generated by gen_rys_tables.py - do not edit.",
+           "// Piecewise-Chebyshev Rys nodes/weights on T in [0, %g), Gauss-Hermite beyond." % TMAX,
+           "#pragma once",
+           "#define RYS_NMAX %d" % NMAX,
+           "#define RYS_NINT %d" % NINT,
+           "#define RYS_DEG %d" % DEG,
+           "#define RYS_H %.17g" % H,
+           "#define RYS_TMAX %.17g" % TMAX,
+           "#define RYS_CHEB_SIZE %d" % o]
+    with open(os.path.join(os.path.dirname(out), "rys_params.h"), "w") as f:
+        f.write("\n".join(hdr) + "\n")
+    lines = ["// Generated by gen_rys_tables.py - do not edit. Data of rys_params.h;",
+             "// define RYS_TABLE_QUALIFIER before including."]
     lines.append("RYS_TABLE_QUALIFIER double RYS_CHEB[RYS_CHEB_SIZE] = {")
     for n in range(1, NMAX + 1):
         lines.append("// n = %d" % n)

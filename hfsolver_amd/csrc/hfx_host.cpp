@@ -1,0 +1,950 @@
+// Host runtime of the direct J/K engine: the C ABI of include/hfx.h.
+//
+// Set-up (hfx_create / hfx_set_geometry):
+//   1. regroup the per-basis-function arrays of gaussints (src/basis.f90:541-567)
+//      into Cartesian shells, as src/gaussians_libint.f90:59-122 recovers shells
+//      by stepping (L+1)(L+2)/2 functions; each function's contraction
+//      coefficients are (shell coefficients) x (per-function scale);
+//   2. all shell pairs (A,B), oriented l_A >= l_B, with their primitive pairs;
+//   3. Schwarz bounds Q_AB = sqrt(max |(ab|ab)|) on the GPU;
+//   4. per pair class, the significant pairs sorted by Q (descending), and for
+//      every class combination the 8x32 tiles of the (bra, ket) plane whose
+//      quartets pass Q_ab Q_cd >= schwarz_threshold, split into an FP64 and an
+//      FP32 share at fp32_threshold.
+// Per Fock build (hfx_fock_2e): P -> scaled symmetric D, one launch per class
+// combination and precision, then J, K symmetrised and unscaled, G = J - K/2.
+#include <hip/hip_runtime.h>
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hfx.h"
+#include "hfx_common.h"
+#include "hfx_internal.h"
+#include "rys.h"
+
+#define RYS_TABLE_QUALIFIER static const
+namespace hfx_host_tables {
+#include "rys_tables.inc"
+}
+#undef RYS_TABLE_QUALIFIER
+
+namespace {
+
+struct Shell {
+    int L;
+    int atom;
+    int f0;       // first basis function
+    int nprim;
+    int prim0;    // into alpha / coef
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct hfx_engine {
+    hfx_opts opts;
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // basis
+    int nbf = 0;
+    int nat = 0;
+    std::vector<int> atno;
+    std::vector<double> xyz;          // 3*nat
+    std::vector<Shell> shells;
+    std::vector<double> alpha, coef;  // shell-level primitives
+    std::vector<double> scale;        // per basis function
+    std::vector<int> bf_atom;
+    // pairs
+    std::vector<ShellPair> pairs;
+    std::vector<PrimPair> prims;
+    std::vector<PrimExp> pexp;
+    std::vector<int> cls_list[HFX_NPAIRCLASS];  // significant pairs, Q descending
+    struct Combo {
+        int cb, ck;
+        int ntiles[2] = {0, 0};
+        int* d_ti0[2] = {nullptr, nullptr};
+        int* d_tj0[2] = {nullptr, nullptr};
+        int* d_kbeg[2] = {nullptr, nullptr};
+        int* d_kend[2] = {nullptr, nullptr};
+        double nq[2] = {0, 0};
+        double nprimq[2] = {0, 0};
+    };
+    std::vector<Combo> combos;
+    double npairs_sig = 0;
+    // device memory
+    ShellPair* d_pairs = nullptr;
+    PrimPair* d_prims = nullptr;
+    PrimExp* d_pexp = nullptr;
+    int* d_cls_list[HFX_NPAIRCLASS] = {nullptr};
+    double* d_scale = nullptr;
+    double* d_D = nullptr;
+    double* d_Jb = nullptr;
+    double* d_Kb = nullptr;
+    double* d_P = nullptr;    // host-pointer entry staging
+    double* d_G = nullptr;
+    double* d_J = nullptr;
+    double* d_K = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void*> geom_allocs;   // freed on geometry change
+};
+
+namespace {
+
+int fail(hfx_t* h, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return 1;
+}
+
+#define HIPCHK(h, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(h, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+int dmalloc(hfx_t* h, void** p, size_t bytes, bool geom) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(h, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    if (geom) h->geom_allocs.push_back(*p);
+    return 0;
+}
+
+template <typename T>
+int upload(hfx_t* h, T** dptr, const std::vector<T>& v, bool geom) {
+    if (dmalloc(h, (void**)dptr, v.size() * sizeof(T), geom)) return 1;
+    if (!v.empty()) HIPCHK(h, hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+void free_geometry(hfx_t* h) {
+    for (void* p : h->geom_allocs) (void)hipFree(p);
+    h->geom_allocs.clear();
+    h->combos.clear();
+    h->d_pairs = nullptr;
+    h->d_prims = nullptr;
+    h->d_pexp = nullptr;
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) {
+        h->d_cls_list[c] = nullptr;
+        h->cls_list[c].clear();
+    }
+}
+
+RysTables host_tables() {
+    return RysTables{hfx_host_tables::RYS_CHEB, hfx_host_tables::RYS_HERM_U, hfx_host_tables::RYS_HERM_W};
+}
+
+void rys_roots_rt(int nr, double T, double* u, double* w) {
+    const RysTables t = host_tables();
+    switch (nr) {
+        case 1: rys_roots<1>(T, u, w, t); break;
+        case 2: rys_roots<2>(T, u, w, t); break;
+        case 3: rys_roots<3>(T, u, w, t); break;
+        case 4: rys_roots<4>(T, u, w, t); break;
+        default: rys_roots<5>(T, u, w, t); break;
+    }
+}
+
+// ---------------------------------------------------------------- basis ---
+int build_shells(hfx_t* h, int nbf, const int* nprim, const int* istart, const double* x, const double* y,
+                 const double* z, const int* l, const int* m, const int* n, int n2, const double* coef,
+                 const double* alpha) {
+    h->shells.clear();
+    h->alpha.clear();
+    h->coef.clear();
+    h->scale.assign(nbf, 1.0);
+    h->bf_atom.assign(nbf, -1);
+    int i = 0;
+    while (i < nbf) {
+        const int L = l[i] + m[i] + n[i];
+        if (L < 0 || L > HFX_LMAX) return fail(h, "basis function %d: angular momentum %d not supported (max %d)", i, L, HFX_LMAX);
+        const int nc = ncart(L);
+        if (i + nc > nbf) return fail(h, "basis function %d: incomplete Cartesian shell", i);
+        const int k = nprim[i];
+        if (k <= 0) return fail(h, "basis function %d: nprim = %d", i, k);
+        for (int c = 0; c < nc; c++) {
+            const int f = i + c;
+            if (l[f] != cart_exp(L, c, 0) || m[f] != cart_exp(L, c, 1) || n[f] != cart_exp(L, c, 2))
+                return fail(h, "basis function %d: powers (%d,%d,%d) out of Cartesian shell order", f, l[f], m[f], n[f]);
+            if (nprim[f] != k || x[f] != x[i] || y[f] != y[i] || z[f] != z[i])
+                return fail(h, "basis function %d: shell members differ in centre or contraction", f);
+            if (istart[f] < 0 || istart[f] + k > n2) return fail(h, "basis function %d: istart out of range", f);
+            for (int p = 0; p < k; p++)
+                if (alpha[istart[f] + p] != alpha[istart[i] + p])
+                    return fail(h, "basis function %d: shell members differ in exponents", f);
+        }
+        // atom of the shell
+        int atom = -1;
+        for (int a = 0; a < h->nat; a++)
+            if (h->xyz[3 * a] == x[i] && h->xyz[3 * a + 1] == y[i] && h->xyz[3 * a + 2] == z[i]) {
+                atom = a;
+                break;
+            }
+        if (atom < 0) return fail(h, "basis function %d: centre is not on any atom", i);
+        Shell s;
+        s.L = L;
+        s.atom = atom;
+        s.f0 = i;
+        s.nprim = k;
+        s.prim0 = (int)h->alpha.size();
+        for (int p = 0; p < k; p++) {
+            h->alpha.push_back(alpha[istart[i] + p]);
+            h->coef.push_back(coef[istart[i] + p]);
+        }
+        // per-function scale: coef_f = scale_f * coef_shell (checked on every primitive)
+        for (int c = 0; c < nc; c++) {
+            const int f = i + c;
+            const double c0 = coef[istart[i]];
+            const double sc = coef[istart[f]] / c0;
+            for (int p = 1; p < k; p++) {
+                const double r = coef[istart[f] + p] - sc * coef[istart[i] + p];
+                if (std::fabs(r) > 1e-12 * std::fabs(coef[istart[f] + p]))
+                    return fail(h, "basis function %d: contraction not proportional within its shell", f);
+            }
+            h->scale[f] = sc;
+            h->bf_atom[f] = atom;
+        }
+        h->shells.push_back(s);
+        i += nc;
+    }
+    h->nbf = nbf;
+    return 0;
+}
+
+// --------------------------------------------------------------- pairs ---
+void build_pairs(hfx_t* h) {
+    h->pairs.clear();
+    h->prims.clear();
+    h->pexp.clear();
+    const int ns = (int)h->shells.size();
+    for (int i = 0; i < ns; i++)
+        for (int j = 0; j <= i; j++) {
+            int A = i, B = j;
+            if (h->shells[A].L < h->shells[B].L) std::swap(A, B);
+            const Shell& sa = h->shells[A];
+            const Shell& sb = h->shells[B];
+            const double* ra = &h->xyz[3 * sa.atom];
+            const double* rb = &h->xyz[3 * sb.atom];
+            ShellPair sp;
+            sp.A = A;
+            sp.B = B;
+            sp.fa = sa.f0;
+            sp.fb = sb.f0;
+            sp.cls = pair_class(sa.L, sb.L);
+            sp.pad = 0;
+            sp.ABx = ra[0] - rb[0];
+            sp.ABy = ra[1] - rb[1];
+            sp.ABz = ra[2] - rb[2];
+            sp.Q = 0;
+            sp.p0 = (int)h->prims.size();
+            const double r2 = sp.ABx * sp.ABx + sp.ABy * sp.ABy + sp.ABz * sp.ABz;
+            for (int pa = 0; pa < sa.nprim; pa++)
+                for (int pb = 0; pb < sb.nprim; pb++) {
+                    const double a = h->alpha[sa.prim0 + pa], b = h->alpha[sb.prim0 + pb];
+                    const double p = a + b;
+                    PrimPair pp;
+                    pp.p = p;
+                    pp.Px = (a * ra[0] + b * rb[0]) / p;
+                    pp.Py = (a * ra[1] + b * rb[1]) / p;
+                    pp.Pz = (a * ra[2] + b * rb[2]) / p;
+                    pp.PAx = pp.Px - ra[0];
+                    pp.PAy = pp.Py - ra[1];
+                    pp.PAz = pp.Pz - ra[2];
+                    pp.K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-a * b / p * r2);
+                    h->prims.push_back(pp);
+                    h->pexp.push_back(PrimExp{a, b});
+                }
+            sp.np = (int)h->prims.size() - sp.p0;
+            h->pairs.push_back(sp);
+        }
+}
+
+// FLOPs of one primitive quartet / one contracted integral of a class
+// combination: the operation count of the un-sliced Rys algorithm of
+// eri_rys.h (frozen model used for the roofline figure).
+struct FlopModel {
+    double per_prim;
+    double per_int;
+};
+
+FlopModel flop_model(int la, int lb, int lc, int ld) {
+    const int LAB = la + lb, LCD = lc + ld, NR = (LAB + LCD) / 2 + 1;
+    const int NA = ncart(la), NB = ncart(lb), NC = ncart(lc), ND = ncart(ld);
+    double f = 0;
+    f += 30;                                   // P-Q, T, rho, prefactor (2 div, 1 sqrt)
+    f += 2.0 * NR * (3.0 * RYS_DEG + 3.0) + 8; // Clenshaw for u_i, w_i
+    double per_root = 16;                      // B00, B10, B01, C00, C00' (x3)
+    double g = 0;
+    g += (LAB > 0 ? 1 : 0) + 3.0 * std::max(0, LAB - 1);
+    for (int mm = 0; mm < LCD; mm++)
+        for (int nn = 0; nn <= LAB; nn++) g += 1 + (mm > 0 ? 3 : 0) + (nn > 0 ? 3 : 0);
+    for (int ib = 1; ib <= lb; ib++) g += 2.0 * (LAB - ib + 1) * (LCD + 1);
+    for (int id = 1; id <= ld; id++) g += 2.0 * (la + 1) * (lb + 1) * (LCD - id + 1);
+    per_root += 3 * g + 1;
+    per_root += 3.0 * NA * NB * NC * ND;       // Ix*Iy, fma with Iz
+    f += NR * per_root;
+    return FlopModel{f, 12.0};
+}
+
+// ------------------------------------------------------------- screening ---
+int compute_schwarz(hfx_t* h) {
+    const int np = (int)h->pairs.size();
+    std::vector<int> all[HFX_NPAIRCLASS];
+    for (int i = 0; i < np; i++) all[h->pairs[i].cls].push_back(i);
+    unsigned long long* d_q2 = nullptr;
+    if (dmalloc(h, (void**)&d_q2, np * sizeof(unsigned long long), false)) return 1;
+    HIPCHK(h, hipMemsetAsync(d_q2, 0, np * sizeof(unsigned long long), h->stream));
+    std::vector<int*> tmp;
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) {
+        if (all[c].empty()) continue;
+        int* d_list = nullptr;
+        if (upload(h, &d_list, all[c], false)) return 1;
+        tmp.push_back(d_list);
+        HIPCHK(h, hfx_launch_schwarz(c, h->d_pairs, h->d_prims, d_list, (int)all[c].size(), d_q2, h->stream));
+    }
+    std::vector<unsigned long long> q2(np);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(q2.data(), d_q2, np * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int* p : tmp) (void)hipFree(p);
+    (void)hipFree(d_q2);
+    for (int i = 0; i < np; i++) {
+        double v;
+        std::memcpy(&v, &q2[i], sizeof v);
+        h->pairs[i].Q = std::sqrt(v);
+    }
+    return 0;
+}
+
+int build_lists(hfx_t* h) {
+    const double thr = h->opts.schwarz_threshold;
+    const double thr32 = h->opts.fp32_threshold;
+    double qmax = 0;
+    for (const ShellPair& p : h->pairs) qmax = std::max(qmax, p.Q);
+    h->npairs_sig = 0;
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) h->cls_list[c].clear();
+    for (int i = 0; i < (int)h->pairs.size(); i++)
+        if (h->pairs[i].Q * qmax >= thr && h->pairs[i].Q > 0) {
+            h->cls_list[h->pairs[i].cls].push_back(i);
+            h->npairs_sig += 1;
+        }
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) {
+        std::stable_sort(h->cls_list[c].begin(), h->cls_list[c].end(),
+                         [&](int a, int b) { return h->pairs[a].Q > h->pairs[b].Q; });
+        if (!h->cls_list[c].empty())
+            if (upload(h, &h->d_cls_list[c], h->cls_list[c], true)) return 1;
+    }
+    h->combos.clear();
+    for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
+        for (int ck = 0; ck <= cb; ck++) {
+            const std::vector<int>& bl = h->cls_list[cb];
+            const std::vector<int>& kl = h->cls_list[ck];
+            if (bl.empty() || kl.empty()) continue;
+            const int nb = (int)bl.size();
+            std::vector<double> kq(kl.size()), knp(kl.size() + 1, 0.0);
+            for (size_t j = 0; j < kl.size(); j++) {
+                kq[j] = h->pairs[kl[j]].Q;
+                knp[j + 1] = knp[j] + h->pairs[kl[j]].np;
+            }
+            // number of kets with Q_i Q_j >= t: prefix of the descending list
+            auto prefix = [&](double qi, double t) -> int {
+                if (qi <= 0) return 0;
+                const double need = t / qi;
+                return (int)(std::upper_bound(kq.begin(), kq.end(), need, [](double v, double e) { return v > e; }) -
+                             kq.begin());
+            };
+            std::vector<int> kmax(nb), k64(nb);
+            for (int i = 0; i < nb; i++) {
+                const double qi = h->pairs[bl[i]].Q;
+                int km = prefix(qi, thr);
+                if (cb == ck) km = std::min(km, i + 1);
+                int k6 = thr32 > 0 ? prefix(qi, thr32) : km;
+                k64[i] = std::min(k6, km);
+                kmax[i] = km;
+            }
+            hfx_engine::Combo cmb;
+            cmb.cb = cb;
+            cmb.ck = ck;
+            FlopModel fm = flop_model(pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck));
+            (void)fm;
+            for (int prec = 0; prec < 2; prec++) {
+                std::vector<int> kb(nb), ke(nb), ti0, tj0;
+                for (int i = 0; i < nb; i++) {
+                    kb[i] = prec == 0 ? 0 : k64[i];
+                    ke[i] = prec == 0 ? k64[i] : kmax[i];
+                    const double nqi = std::max(0, ke[i] - kb[i]);
+                    cmb.nq[prec] += nqi;
+                    if (nqi > 0) cmb.nprimq[prec] += h->pairs[bl[i]].np * (knp[ke[i]] - knp[kb[i]]);
+                }
+                for (int r0 = 0; r0 < nb; r0 += HFX_TILE_ROWS) {
+                    int lo = 1 << 30, hi = 0;
+                    for (int r = r0; r < std::min(nb, r0 + HFX_TILE_ROWS); r++)
+                        if (ke[r] > kb[r]) {
+                            lo = std::min(lo, kb[r]);
+                            hi = std::max(hi, ke[r]);
+                        }
+                    if (hi <= lo) continue;
+                    for (int j0 = (lo / HFX_TILE_COLS) * HFX_TILE_COLS; j0 < hi; j0 += HFX_TILE_COLS) {
+                        ti0.push_back(r0);
+                        tj0.push_back(j0);
+                    }
+                }
+                cmb.ntiles[prec] = (int)ti0.size();
+                if (cmb.ntiles[prec] > 0) {
+                    if (upload(h, &cmb.d_ti0[prec], ti0, true)) return 1;
+                    if (upload(h, &cmb.d_tj0[prec], tj0, true)) return 1;
+                    if (upload(h, &cmb.d_kbeg[prec], kb, true)) return 1;
+                    if (upload(h, &cmb.d_kend[prec], ke, true)) return 1;
+                }
+            }
+            h->combos.push_back(cmb);
+        }
+    if (h->opts.verbose) {
+        double q64 = 0, q32 = 0;
+        for (auto& c : h->combos) {
+            q64 += c.nq[0];
+            q32 += c.nq[1];
+        }
+        fprintf(stderr, "hfx: %d shells, %zu pairs (%.0f significant), quartets fp64 %.4g fp32 %.4g\n",
+                (int)h->shells.size(), h->pairs.size(), h->npairs_sig, q64, q32);
+    }
+    return 0;
+}
+
+int setup_geometry(hfx_t* h) {
+    free_geometry(h);
+    build_pairs(h);
+    if (upload(h, &h->d_pairs, h->pairs, true)) return 1;
+    if (upload(h, &h->d_prims, h->prims, true)) return 1;
+    if (upload(h, &h->d_pexp, h->pexp, true)) return 1;
+    if (compute_schwarz(h)) return 1;
+    HIPCHK(h, hipMemcpy(h->d_pairs, h->pairs.data(), h->pairs.size() * sizeof(ShellPair), hipMemcpyHostToDevice));
+    return build_lists(h);
+}
+
+// ------------------------------------------------------ one-electron ----
+// 1-D overlap table S[i][j] (i <= imax, j <= jmax) of a primitive pair, without
+// the exp / pi factors: S(0,0) = 1, Obara-Saika recurrences.
+void os_overlap_1d(double PA, double PB, double oo2p, int imax, int jmax, double S[8][8]) {
+    S[0][0] = 1.0;
+    for (int i = 0; i <= imax; i++)
+        for (int j = 0; j <= jmax; j++) {
+            if (i == 0 && j == 0) continue;
+            double v;
+            if (i > 0) {
+                v = PA * S[i - 1][j];
+                if (i > 1) v += (i - 1) * oo2p * S[i - 2][j];
+                if (j > 0) v += j * oo2p * S[i - 1][j - 1];
+            } else {
+                v = PB * S[i][j - 1];
+                if (j > 1) v += (j - 1) * oo2p * S[i][j - 2];
+            }
+            S[i][j] = v;
+        }
+}
+
+// Nuclear-attraction 2-D table for one Rys root: I[i][j] for i <= imax, j <= jmax.
+void rys_nuc_1d(double c00, double b10, double AB, int imax, int jmax, double g0, double I[8][8]) {
+    double G[16];
+    const int nmax = imax + jmax;
+    G[0] = g0;
+    if (nmax > 0) G[1] = c00 * g0;
+    for (int k = 1; k < nmax; k++) G[k + 1] = c00 * G[k] + k * b10 * G[k - 1];
+    double H[16][8];
+    for (int k = 0; k <= nmax; k++) H[k][0] = G[k];
+    for (int j = 1; j <= jmax; j++)
+        for (int k = 0; k <= nmax - j; k++) H[k][j] = H[k + 1][j - 1] + AB * H[k][j - 1];
+    for (int i = 0; i <= imax; i++)
+        for (int j = 0; j <= jmax; j++) I[i][j] = H[i][j];
+}
+
+// Primitive-pair blocks of S, T, V (and optionally their derivatives with
+// respect to the centre of A) for shells A, B. da: derivative order 0 or 1.
+struct OneEBlock {
+    double S[6][6], T[6][6], V[6][6];
+    double dS[3][6][6], dT[3][6][6], dV[3][6][6];  // d/dA
+};
+
+// With deriv, pw (density block of A x B, scaled) and gop (3*nat) given, the
+// operator (Hellmann-Feynman) part -2 sum_ab P_ab d/dA <a|V_c|b> is added to
+// gop[3c + d] (d/dC = -(d/dA + d/dB); the d/dB half is the (B, A) visit's).
+void one_e_block(const hfx_t* h, int A, int B, bool deriv, OneEBlock& out, const double (*pw)[6] = nullptr,
+                 double* gop = nullptr) {
+    const Shell& sa = h->shells[A];
+    const Shell& sb = h->shells[B];
+    const int la = sa.L, lb = sb.L, na = ncart(la), nb = ncart(lb);
+    const double* ra = &h->xyz[3 * sa.atom];
+    const double* rb = &h->xyz[3 * sb.atom];
+    const double AB[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+    const double r2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+    std::memset(out.S, 0, sizeof out.S);
+    std::memset(out.T, 0, sizeof out.T);
+    std::memset(out.V, 0, sizeof out.V);
+    std::memset(out.dS, 0, sizeof out.dS);
+    std::memset(out.dT, 0, sizeof out.dT);
+    std::memset(out.dV, 0, sizeof out.dV);
+    const int ea = la + (deriv ? 1 : 0);  // highest power needed on A
+    for (int pa = 0; pa < sa.nprim; pa++)
+        for (int pb = 0; pb < sb.nprim; pb++) {
+            const double a = h->alpha[sa.prim0 + pa], b = h->alpha[sb.prim0 + pb];
+            const double p = a + b, oo2p = 0.5 / p;
+            const double K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-a * b / p * r2);
+            double P[3], PA[3], PB[3];
+            for (int d = 0; d < 3; d++) {
+                P[d] = (a * ra[d] + b * rb[d]) / p;
+                PA[d] = P[d] - ra[d];
+                PB[d] = P[d] - rb[d];
+            }
+            // overlap / kinetic
+            double S1[3][8][8];
+            for (int d = 0; d < 3; d++) os_overlap_1d(PA[d], PB[d], oo2p, ea, lb + 2, S1[d]);
+            const double s0 = std::pow(HFX_PI / p, 1.5) * K;
+            auto Tk = [&](int d, int i, int j) {  // 1-D kinetic (-1/2 d2/dx2 on b)
+                double v = -2.0 * b * b * S1[d][i][j + 2] + b * (2 * j + 1) * S1[d][i][j];
+                if (j >= 2) v -= 0.5 * j * (j - 1) * S1[d][i][j - 2];
+                return v;
+            };
+            auto ST = [&](int ax, int ay, int az, int bx, int by, int bz, double& s, double& t) {
+                if (ax < 0 || ay < 0 || az < 0) {
+                    s = t = 0;
+                    return;
+                }
+                const double sx = S1[0][ax][bx], sy = S1[1][ay][by], sz = S1[2][az][bz];
+                s = s0 * sx * sy * sz;
+                t = s0 * (Tk(0, ax, bx) * sy * sz + sx * Tk(1, ay, by) * sz + sx * sy * Tk(2, az, bz));
+            };
+            for (int ia = 0; ia < na; ia++) {
+                const int ax = cart_exp(la, ia, 0), ay = cart_exp(la, ia, 1), az = cart_exp(la, ia, 2);
+                for (int ib = 0; ib < nb; ib++) {
+                    const int bx = cart_exp(lb, ib, 0), by = cart_exp(lb, ib, 1), bz = cart_exp(lb, ib, 2);
+                    double s, t;
+                    ST(ax, ay, az, bx, by, bz, s, t);
+                    out.S[ia][ib] += s;
+                    out.T[ia][ib] += t;
+                    if (deriv) {
+                        const int e[3] = {ax, ay, az};
+                        for (int d = 0; d < 3; d++) {
+                            int up[3] = {ax, ay, az}, dn[3] = {ax, ay, az};
+                            up[d]++;
+                            dn[d]--;
+                            double su, tu, sd = 0, td = 0;
+                            ST(up[0], up[1], up[2], bx, by, bz, su, tu);
+                            if (e[d] > 0) ST(dn[0], dn[1], dn[2], bx, by, bz, sd, td);
+                            out.dS[d][ia][ib] += 2 * a * su - e[d] * sd;
+                            out.dT[d][ia][ib] += 2 * a * tu - e[d] * td;
+                        }
+                    }
+                }
+            }
+            // nuclear attraction by Rys quadrature
+            const int nr = (ea + lb) / 2 + 1;
+            for (int c = 0; c < h->nat; c++) {
+                const double* rc = &h->xyz[3 * c];
+                const double PC[3] = {P[0] - rc[0], P[1] - rc[1], P[2] - rc[2]};
+                const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                double u[RYS_NMAX], w[RYS_NMAX];
+                rys_roots_rt(nr, T, u, w);
+                const double pref = -h->atno[c] * 2.0 * HFX_PI / p * K;
+                for (int r = 0; r < nr; r++) {
+                    double I[3][8][8];
+                    const double b10 = (1.0 - u[r]) * oo2p;
+                    for (int d = 0; d < 3; d++)
+                        rys_nuc_1d(PA[d] - u[r] * PC[d], b10, AB[d], ea, lb, d == 2 ? w[r] * pref : 1.0, I[d]);
+                    auto val = [&](int ax, int ay, int az, int bx, int by, int bz) {
+                        if (ax < 0 || ay < 0 || az < 0) return 0.0;
+                        return I[0][ax][bx] * I[1][ay][by] * I[2][az][bz];
+                    };
+                    for (int ia = 0; ia < na; ia++) {
+                        const int ax = cart_exp(la, ia, 0), ay = cart_exp(la, ia, 1), az = cart_exp(la, ia, 2);
+                        for (int ib = 0; ib < nb; ib++) {
+                            const int bx = cart_exp(lb, ib, 0), by = cart_exp(lb, ib, 1), bz = cart_exp(lb, ib, 2);
+                            out.V[ia][ib] += val(ax, ay, az, bx, by, bz);
+                            if (deriv) {
+                                const int e[3] = {ax, ay, az};
+                                for (int d = 0; d < 3; d++) {
+                                    int up[3] = {ax, ay, az}, dn[3] = {ax, ay, az};
+                                    up[d]++;
+                                    dn[d]--;
+                                    const double dv = 2 * a * val(up[0], up[1], up[2], bx, by, bz) -
+                                                      (e[d] > 0 ? e[d] * val(dn[0], dn[1], dn[2], bx, by, bz) : 0.0);
+                                    out.dV[d][ia][ib] += dv;
+                                    if (gop) gop[3 * c + d] -= 2.0 * pw[ia][ib] * dv;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+}
+
+}  // namespace
+
+// =================================================================== ABI ===
+extern "C" {
+
+int hfx_default_opts(hfx_opts* o) {
+    if (!o) return 1;
+    o->schwarz_threshold = 1e-12;
+    o->fp32_threshold = 0.0;
+    o->device = 0;
+    o->verbose = 0;
+    return 0;
+}
+
+int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, const double* y, const double* z,
+               const int* l, const int* m, const int* n, int n2, const double* coef, const double* alpha, int nat,
+               const int* atno, const double* xyz, const hfx_opts* o, hfx_t** hout) {
+    if (!hout) return 1;
+    *hout = nullptr;
+    hfx_t* h = new hfx_t();
+    *hout = h;
+    if (o)
+        h->opts = *o;
+    else
+        hfx_default_opts(&h->opts);
+    if (nbf <= 0 || !nprim || !istart || !x || !y || !z || !l || !m || !n || !coef || !alpha || nat <= 0 || !atno ||
+        !xyz)
+        return fail(h, "hfx_create: invalid arguments (nbf=%d nat=%d)", nbf, nat);
+    if (!(h->opts.schwarz_threshold >= 0) || !(h->opts.fp32_threshold >= 0))
+        return fail(h, "hfx_create: thresholds must be >= 0");
+    h->nat = nat;
+    h->atno.assign(atno, atno + nat);
+    h->xyz.assign(xyz, xyz + 3 * nat);
+    if (build_shells(h, nbf, nprim, istart, x, y, z, l, m, n, n2, coef, alpha)) return 1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(h, "hfx_create: no HIP device available");
+    if (h->opts.device < 0 || h->opts.device >= ndev) return fail(h, "hfx_create: device %d out of range", h->opts.device);
+    h->device = h->opts.device;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreate(&h->ev0));
+    HIPCHK(h, hipEventCreate(&h->ev1));
+    const size_t nn = (size_t)nbf * nbf;
+    if (upload(h, &h->d_scale, h->scale, false)) return 1;
+    if (dmalloc(h, (void**)&h->d_D, nn * sizeof(double), false)) return 1;
+    if (dmalloc(h, (void**)&h->d_Jb, nn * sizeof(double), false)) return 1;
+    if (dmalloc(h, (void**)&h->d_Kb, nn * sizeof(double), false)) return 1;
+    return setup_geometry(h);
+}
+
+int hfx_set_geometry(hfx_t* h, const double* xyz) {
+    if (!h || !xyz) return 1;
+    HIPCHK(h, hipSetDevice(h->device));
+    h->xyz.assign(xyz, xyz + 3 * h->nat);
+    return setup_geometry(h);
+}
+
+int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, double* dK, hfx_stats* st) {
+    if (!h || !dP) return h ? fail(h, "hfx_fock_2e_device: null density") : 1;
+    HIPCHK(h, hipSetDevice(h->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const int n = h->nbf;
+    HIPCHK(h, hfx_launch_prep(dP, h->d_scale, h->d_D, h->d_Jb, h->d_Kb, n, h->stream));
+    JKArgs a;
+    a.pairs = h->d_pairs;
+    a.prims = h->d_prims;
+    a.D = h->d_D;
+    a.J = h->d_Jb;
+    a.K = h->d_Kb;
+    a.n = n;
+    double q64 = 0, q32 = 0, f64 = 0, f32 = 0;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    for (const auto& c : h->combos) {
+        a.bra_list = h->d_cls_list[c.cb];
+        a.ket_list = h->d_cls_list[c.ck];
+        a.nbra = (int)h->cls_list[c.cb].size();
+        a.same = c.cb == c.ck;
+        const FlopModel fm =
+            flop_model(pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck));
+        const double nint = (double)ncart(pair_class_la(c.cb)) * ncart(pair_class_lb(c.cb)) *
+                            ncart(pair_class_la(c.ck)) * ncart(pair_class_lb(c.ck));
+        for (int prec = 0; prec < 2; prec++) {
+            if (c.ntiles[prec] == 0) continue;
+            a.tile_i0 = c.d_ti0[prec];
+            a.tile_j0 = c.d_tj0[prec];
+            a.kbeg = c.d_kbeg[prec];
+            a.kend = c.d_kend[prec];
+            HIPCHK(h, hfx_launch_jk(c.cb, c.ck, prec == 1, a, c.ntiles[prec], h->stream));
+            const double fl = c.nprimq[prec] * fm.per_prim + c.nq[prec] * nint * fm.per_int;
+            if (prec == 0) {
+                q64 += c.nq[0];
+                f64 += fl;
+            } else {
+                q32 += c.nq[1];
+                f32 += fl;
+            }
+        }
+    }
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    HIPCHK(h, hfx_launch_finish(h->d_Jb, h->d_Kb, h->d_scale, dG, dJ, dK, n, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (st) {
+        st->jk_seconds = 1e-3 * ms;
+        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        st->quartets_fp64 = q64;
+        st->quartets_fp32 = q32;
+        st->flops_fp64 = f64;
+        st->flops_fp32 = f32;
+    }
+    return 0;
+}
+
+int hfx_fock_2e(hfx_t* h, const double* P, double* G, double* J, double* K, hfx_stats* st) {
+    if (!h || !P) return h ? fail(h, "hfx_fock_2e: null density") : 1;
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t nn = (size_t)h->nbf * h->nbf;
+    if (!h->d_P) {
+        if (dmalloc(h, (void**)&h->d_P, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_G, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_J, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_K, nn * sizeof(double), false)) return 1;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(h, hipMemcpy(h->d_P, P, nn * sizeof(double), hipMemcpyHostToDevice));
+    if (hfx_fock_2e_device(h, h->d_P, G ? h->d_G : nullptr, J ? h->d_J : nullptr, K ? h->d_K : nullptr, st)) return 1;
+    if (G) HIPCHK(h, hipMemcpy(G, h->d_G, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (J) HIPCHK(h, hipMemcpy(J, h->d_J, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (K) HIPCHK(h, hipMemcpy(K, h->d_K, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int hfx_one_electron(hfx_t* h, double* S, double* T, double* V) {
+    if (!h) return 1;
+    const int ns = (int)h->shells.size();
+    const int n = h->nbf;
+    std::vector<std::pair<int, int>> sp;
+    for (int i = 0; i < ns; i++)
+        for (int j = 0; j <= i; j++) sp.push_back({i, j});
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long k = 0; k < (long)sp.size(); k++) {
+        const int A = sp[k].first, B = sp[k].second;
+        OneEBlock blk;
+        one_e_block(h, A, B, false, blk);
+        const Shell& sa = h->shells[A];
+        const Shell& sb = h->shells[B];
+        for (int ia = 0; ia < ncart(sa.L); ia++)
+            for (int ib = 0; ib < ncart(sb.L); ib++) {
+                const int fi = sa.f0 + ia, fj = sb.f0 + ib;
+                const double sc = h->scale[fi] * h->scale[fj];
+                const size_t ij = (size_t)fi + (size_t)fj * n, ji = (size_t)fj + (size_t)fi * n;
+                if (S) S[ij] = S[ji] = sc * blk.S[ia][ib];
+                if (T) {
+                    // the kinetic operator acts on b; symmetric in exact arithmetic
+                    T[ij] = T[ji] = sc * blk.T[ia][ib];
+                }
+                if (V) V[ij] = V[ji] = sc * blk.V[ia][ib];
+            }
+    }
+    return 0;
+}
+
+int hfx_grad_1e(hfx_t* h, const double* P, const double* W, double* g) {
+    if (!h || !P || !W || !g) return h ? fail(h, "hfx_grad_1e: null argument") : 1;
+    const int ns = (int)h->shells.size();
+    const int n = h->nbf, nat = h->nat;
+    std::vector<double> acc(3 * nat, 0.0);
+#pragma omp parallel
+    {
+        std::vector<double> gl(3 * nat, 0.0);
+        OneEBlock blk;
+#pragma omp for schedule(dynamic, 4)
+        for (long k = 0; k < (long)ns * ns; k++) {
+            const int A = (int)(k / ns), B = (int)(k % ns);
+            const Shell& sa = h->shells[A];
+            const Shell& sb = h->shells[B];
+            double pw[6][6];
+            for (int ia = 0; ia < ncart(sa.L); ia++)
+                for (int ib = 0; ib < ncart(sb.L); ib++) {
+                    const int fi = sa.f0 + ia, fj = sb.f0 + ib;
+                    pw[ia][ib] = P[(size_t)fi + (size_t)fj * n] * h->scale[fi] * h->scale[fj];
+                }
+            one_e_block(h, A, B, true, blk, pw, gl.data());
+            for (int ia = 0; ia < ncart(sa.L); ia++)
+                for (int ib = 0; ib < ncart(sb.L); ib++) {
+                    const int fi = sa.f0 + ia, fj = sb.f0 + ib;
+                    const double sc = h->scale[fi] * h->scale[fj];
+                    const double pij = P[(size_t)fi + (size_t)fj * n] * sc, wij = W[(size_t)fi + (size_t)fj * n] * sc;
+                    for (int d = 0; d < 3; d++) {
+                        // derivative on the centre of function i; the factor 2 accounts
+                        // for the equal derivative on the centre of j (symmetric P, W)
+                        gl[3 * sa.atom + d] +=
+                            2.0 * (pij * (blk.dT[d][ia][ib] + blk.dV[d][ia][ib]) - wij * blk.dS[d][ia][ib]);
+                    }
+                }
+        }
+#pragma omp critical
+        for (int t = 0; t < 3 * nat; t++) acc[t] += gl[t];
+    }
+    for (int t = 0; t < 3 * nat; t++) g[t] = acc[t];
+    return 0;
+}
+
+int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
+    (void)P;
+    (void)g;
+    return h ? fail(h, "hfx_grad_2e: not available in this build") : 1;
+}
+
+int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
+    if (!h || nq < 0 || (nq > 0 && (!shells || !out))) return h ? fail(h, "hfx_eri_shells: invalid arguments") : 1;
+    if (nq == 0) return 0;
+    HIPCHK(h, hipSetDevice(h->device));
+    const int ns = (int)h->shells.size();
+    // map each requested quartet onto oriented pairs
+    auto pair_index = [&](int i, int j) {  // index of unordered shell pair in h->pairs (built i >= j)
+        if (i < j) std::swap(i, j);
+        return (long)i * (i + 1) / 2 + j;
+    };
+    std::vector<int> bra(nq), ket(nq);
+    std::vector<long long> off(nq), hoff(nq);
+    std::vector<int> perm(nq);  // bit0: bra swapped within pair, bit1: ket swapped, bit2: bra<->ket
+    long long tot = 0;
+    for (int q = 0; q < nq; q++) {
+        int A = shells[4 * q], B = shells[4 * q + 1], C = shells[4 * q + 2], D = shells[4 * q + 3];
+        if (A < 0 || B < 0 || C < 0 || D < 0 || A >= ns || B >= ns || C >= ns || D >= ns)
+            return fail(h, "hfx_eri_shells: shell index out of range in quartet %d", q);
+        int p1 = (int)pair_index(A, B), p2 = (int)pair_index(C, D);
+        int pm = 0;
+        if (h->pairs[p1].A != A) pm |= 1;
+        if (h->pairs[p2].A != C) pm |= 2;
+        if (h->pairs[p1].cls < h->pairs[p2].cls) {
+            std::swap(p1, p2);
+            pm |= 4;
+        }
+        bra[q] = p1;
+        ket[q] = p2;
+        perm[q] = pm;
+        off[q] = tot;
+        hoff[q] = tot;
+        tot += (long long)ncart(h->shells[A].L) * ncart(h->shells[B].L) * ncart(h->shells[C].L) *
+               ncart(h->shells[D].L);
+    }
+    int *d_bra = nullptr, *d_ket = nullptr, *d_q = nullptr;
+    long long* d_off = nullptr;
+    double* d_out = nullptr;
+    if (upload(h, &d_bra, bra, false) || upload(h, &d_ket, ket, false) || upload(h, &d_off, off, false)) return 1;
+    if (dmalloc(h, (void**)&d_out, tot * sizeof(double), false)) return 1;
+    // group by class combination
+    std::vector<int> order(nq);
+    for (int q = 0; q < nq; q++) order[q] = q;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        const int kx = h->pairs[bra[x]].cls * 8 + h->pairs[ket[x]].cls, ky = h->pairs[bra[y]].cls * 8 + h->pairs[ket[y]].cls;
+        return kx < ky;
+    });
+    if (upload(h, &d_q, order, false)) return 1;
+    int s = 0;
+    while (s < nq) {
+        const int cb = h->pairs[bra[order[s]]].cls, ck = h->pairs[ket[order[s]]].cls;
+        int e = s;
+        while (e < nq && h->pairs[bra[order[e]]].cls == cb && h->pairs[ket[order[e]]].cls == ck) e++;
+        BlockArgs ba;
+        ba.pairs = h->d_pairs;
+        ba.prims = h->d_prims;
+        ba.bra = d_bra;
+        ba.ket = d_ket;
+        ba.off = d_off;
+        ba.qidx = d_q + s;
+        ba.nq = e - s;
+        ba.out = d_out;
+        HIPCHK(h, hfx_launch_block(cb, ck, ba, h->stream));
+        s = e;
+    }
+    std::vector<double> tmp(tot);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(tmp.data(), d_out, tot * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(d_bra);
+    (void)hipFree(d_ket);
+    (void)hipFree(d_off);
+    (void)hipFree(d_out);
+    (void)hipFree(d_q);
+    // undo the orientation: kernel layout is (a' b' | c' d') of the oriented pairs
+    for (int q = 0; q < nq; q++) {
+        const int A = shells[4 * q], B = shells[4 * q + 1], C = shells[4 * q + 2], D = shells[4 * q + 3];
+        const int nA = ncart(h->shells[A].L), nB = ncart(h->shells[B].L), nC = ncart(h->shells[C].L),
+                  nD = ncart(h->shells[D].L);
+        const ShellPair& pb = h->pairs[bra[q]];
+        const ShellPair& pk = h->pairs[ket[q]];
+        const int n1 = ncart(h->shells[pb.A].L), n2 = ncart(h->shells[pb.B].L), n3 = ncart(h->shells[pk.A].L),
+                  n4 = ncart(h->shells[pk.B].L);
+        const double* src = &tmp[off[q]];
+        double* dst = out + hoff[q];
+        for (int a = 0; a < nA; a++)
+            for (int b = 0; b < nB; b++)
+                for (int c = 0; c < nC; c++)
+                    for (int d = 0; d < nD; d++) {
+                        // (ab|cd) in request order -> oriented indices
+                        int x1 = a, x2 = b, x3 = c, x4 = d;
+                        if (perm[q] & 1) std::swap(x1, x2);
+                        if (perm[q] & 2) std::swap(x3, x4);
+                        if (perm[q] & 4) {
+                            std::swap(x1, x3);
+                            std::swap(x2, x4);
+                        }
+                        const double sc = h->scale[h->shells[A].f0 + a] * h->scale[h->shells[B].f0 + b] *
+                                          h->scale[h->shells[C].f0 + c] * h->scale[h->shells[D].f0 + d];
+                        dst[((a * nB + b) * nC + c) * nD + d] = sc * src[((x1 * n2 + x2) * n3 + x3) * n4 + x4];
+                    }
+    }
+    return 0;
+}
+
+int hfx_shells(const hfx_t* h, int* nshell, int* first_bf, int* L, int* atom) {
+    if (!h || !nshell) return 1;
+    *nshell = (int)h->shells.size();
+    for (size_t i = 0; i < h->shells.size(); i++) {
+        if (first_bf) first_bf[i] = h->shells[i].f0;
+        if (L) L[i] = h->shells[i].L;
+        if (atom) atom[i] = h->shells[i].atom;
+    }
+    return 0;
+}
+
+int hfx_screening(const hfx_t* h, double* npairs, double* nq64, double* nq32) {
+    if (!h) return 1;
+    double a = 0, b = 0;
+    for (const auto& c : h->combos) {
+        a += c.nq[0];
+        b += c.nq[1];
+    }
+    if (npairs) *npairs = h->npairs_sig;
+    if (nq64) *nq64 = a;
+    if (nq32) *nq32 = b;
+    return 0;
+}
+
+const char* hfx_last_error(const hfx_t* h) { return h ? h->err.c_str() : "null handle"; }
+
+void hfx_destroy(hfx_t* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    free_geometry(h);
+    void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

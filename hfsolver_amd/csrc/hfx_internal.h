@@ -1,0 +1,58 @@
+// Interface between the host runtime (hfx_host.cpp) and the HIP kernels
+// (hfx_kernels.hip). Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hfx_common.h"
+
+// Arguments of one J/K launch: bra pairs of class X (rows), ket pairs of class
+// Y (columns), X >= Y; both lists sorted by Schwarz bound, descending, so the
+// significant kets of a row form a prefix. Row i evaluates columns
+// [kbeg[i], kend[i]) (one precision's share of that prefix; for X == Y the
+// prefix is cut at the diagonal). Tiles: 8 rows x 32 columns, 256 threads.
+struct JKArgs {
+    const ShellPair* pairs;
+    const PrimPair* prims;
+    const int* bra_list;
+    const int* ket_list;
+    const int* tile_i0;
+    const int* tile_j0;
+    const int* kbeg;
+    const int* kend;
+    int nbra;
+    int same;   // X == Y
+    const double* D;  // scaled, symmetrised density, n x n
+    double* J;        // unsymmetrised accumulators, n x n
+    double* K;
+    int n;
+};
+
+// Arguments of the explicit-quartet kernel: quartet q uses pairs bra[q], ket[q]
+// (already oriented l_A >= l_B, l_C >= l_D, class(bra) >= class(ket)) and
+// writes its block at out + off[q] in the (a, b, c, d) order of the oriented
+// pairs.
+struct BlockArgs {
+    const ShellPair* pairs;
+    const PrimPair* prims;
+    const int* bra;
+    const int* ket;
+    const long long* off;
+    const int* qidx;   // quartet ids of this class combination
+    int nq;
+    double* out;
+};
+
+static constexpr int HFX_TILE_ROWS = 8;
+static constexpr int HFX_TILE_COLS = 32;
+
+hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, int ntiles, hipStream_t s);
+hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
+// Q2[pair] = max over the diagonal integrals (ab|ab) of the pair (squared bound)
+hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,
+                              unsigned long long* q2bits, hipStream_t s);
+hipError_t hfx_launch_prep(const double* P, const double* scale, double* D, double* Jb, double* Kb, int n,
+                           hipStream_t s);
+hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* scale, double* G, double* J, double* K,
+                             int n, hipStream_t s);
+int hfx_nsplit(int cls_bra, int cls_ket);

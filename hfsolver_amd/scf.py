@@ -1,0 +1,134 @@
+"""Closed-shell RHF driver over the GPU two-electron engine.
+
+rhf_gauss mirrors hfsolver's driver of the same name (src/drivers.f90:12-51)
+and doscf (src/scf.f90:72-136): same arguments (atomic numbers, bohr
+coordinates, Nscf, mixing alpha, tolE, tolP) and the same semantics:
+  * P starts at 0, so the first Fock matrix is the core Hamiltonian (scf.f90:107);
+  * each iteration: G = G(P) [here: one hfx_fock_2e call instead of
+    formG(P, TT)], F = H + G, E0 = 1/2 sum P(H+F), F C = S C lambda by LAPACK
+    dsygvd on the lower triangle (linalg.f90:241-280), P = 2 C_occ C_occ^T;
+  * dP = sqrt(sum (P - Pold)^2 / 4); converged when the iteration count
+    exceeds 3, dP < tolP and the spread of the last four E0 is below tolE;
+    otherwise P = (1 - alpha) Pold + alpha P; failure to converge in Nscf
+    iterations raises (the reference stops with exit code 1);
+  * on return E0, lambda and C belong to the Fock matrix built from the last
+    input density, P is the last output density; Enuc = sum Z_i Z_j / R_ij,
+    Ekin = sum P T, Etot = E0 + Enuc.
+Additions with no counterpart in the reference: optional DIIS extrapolation
+of F (Pulay, error FPS - SPF), a warm-start density P0, and the FP32
+threshold of the mixed-precision engine.
+"""
+import math
+
+import numpy as np
+import scipy.linalg
+
+from .basis import Basis
+from .engine import Engine
+
+
+class RHFResult:
+    """Outputs of rhf_gauss (names follow the reference's arguments)."""
+
+
+def nuclear_energy(atno, xyz):
+    xyz = np.asarray(xyz, dtype=float).reshape(-1, 3)
+    E = 0.0
+    for i in range(len(atno)):
+        for j in range(i):
+            E += float(atno[i]) * float(atno[j]) / math.sqrt(float(np.sum((xyz[i] - xyz[j]) ** 2)))
+    return E
+
+
+def nuclear_gradient(atno, xyz):
+    xyz = np.asarray(xyz, dtype=float).reshape(-1, 3)
+    g = np.zeros_like(xyz)
+    for i in range(len(atno)):
+        for j in range(len(atno)):
+            if i != j:
+                d = xyz[i] - xyz[j]
+                g[i] -= float(atno[i]) * float(atno[j]) * d / np.linalg.norm(d) ** 3
+    return g
+
+
+def _diis_extrapolate(Fs, errs):
+    m = len(Fs)
+    B = -np.ones((m + 1, m + 1))
+    B[m, m] = 0.0
+    for i in range(m):
+        for j in range(m):
+            B[i, j] = float(np.vdot(errs[i], errs[j]))
+    rhs = np.zeros(m + 1)
+    rhs[m] = -1.0
+    try:
+        c = np.linalg.solve(B, rhs)[:m]
+    except np.linalg.LinAlgError:
+        return Fs[-1]
+    return sum(ci * Fi for ci, Fi in zip(c, Fs))
+
+
+def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=False, diis_size=8, P0=None,
+          callback=None):
+    """The SCF loop of src/scf.f90:72-136 with G(P) from `fock_2e`."""
+    n = H.shape[0]
+    P = np.zeros((n, n)) if P0 is None else np.array(P0, dtype=float)
+    Escf, Fs, errs = [], [], []
+    for it in range(1, Nscf + 1):
+        G = fock_2e(P)
+        F = H + G
+        E0 = 0.5 * float(np.sum(P * (H + F)))
+        Escf.append(E0)
+        if callback is not None:
+            callback(it, E0, P, G)
+        Fd = F
+        if diis:
+            err = F @ P @ S - S @ P @ F
+            Fs.append(F)
+            errs.append(err)
+            if len(Fs) > diis_size:
+                Fs.pop(0)
+                errs.pop(0)
+            if len(Fs) >= 2:
+                Fd = _diis_extrapolate(Fs, errs)
+        lam, C = scipy.linalg.eigh(Fd, S, lower=True, driver="gvd")
+        Pold = P
+        P = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+        dP = math.sqrt(float(np.sum((P - Pold) ** 2)) / 4)
+        if it > 3:
+            w = Escf[it - 4:it]
+            if dP < tolP and max(w) - min(w) < tolE:
+                return dict(C=C, P=P, Pin=Pold, lam=lam, E0=E0, F=F, G=G, iterations=it, dP=dP, E_iter=Escf)
+        if not diis:
+            P = (1 - alpha) * Pold + alpha * P
+    raise RuntimeError("SCF did not converge in %d iterations." % Nscf)
+
+
+def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=False, P0=None, fp32_threshold=0.0,
+              schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0):
+    """Closed-shell RHF of a neutral molecule (src/drivers.f90:12-51)."""
+    basis = Basis(atno, xyz) if engine is None else engine.basis
+    eng = engine if engine is not None else Engine(basis, schwarz_threshold=schwarz_threshold,
+                                                   fp32_threshold=fp32_threshold, device=device)
+    try:
+        S, T, V = eng.one_electron()
+        H = T + V
+        nelec = int(np.sum(atno))
+        r = doscf(H, S, nelec // 2, lambda P: eng.fock_2e(P)[0], Nscf, alpha, tolE, tolP, diis=diis, P0=P0)
+    except Exception:
+        if engine is None and not keep_engine:
+            eng.close()
+        raise
+    out = RHFResult()
+    for k, v in r.items():
+        setattr(out, k, v)
+    out.S, out.T, out.V, out.H = S, T, V, H
+    out.Enuc = nuclear_energy(atno, xyz)
+    out.Ekin = float(np.sum(out.P * T))
+    out.Etot = out.E0 + out.Enuc
+    out.basis = basis
+    if engine is None and not keep_engine:
+        eng.close()
+        out.engine = None
+    else:
+        out.engine = eng
+    return out

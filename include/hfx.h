@@ -1,0 +1,137 @@
+/*
+ * hfx.h - C ABI of the MI355X direct-SCF two-electron engine.
+ *
+ * Drop-in boundary for hfsolver's closed-shell RHF path. In the reference the
+ * two-electron work is split between
+ *   - the integral plug-in `getints2` (src/gaussians_hfsolver.f90:13-34), which
+ *     calls C `getInts` (src/qc.h:35-39, src/qc.c:95-132) once per SCF to fill
+ *     the packed int2 array, and
+ *   - `G = formG(P, TT)` (src/scf.f90:26-37, call site src/scf.f90:109),
+ *     evaluated every SCF iteration from the stored n^4 tensor TT.
+ * This engine replaces both by one direct call per iteration: hfx_fock_2e
+ * evaluates the Schwarz-screened electron-repulsion integrals on the GPU and
+ * contracts them with the density on the fly (they are never stored).
+ *
+ * Conventions (same as the reference's C kernels):
+ *   - basis description: the per-basis-function arrays gaussints already holds
+ *     (src/basis.f90:541-567): nprim[nbf], istart[nbf] (0-BASED, as
+ *     gaussints passes istart-1), centre x/y/z[nbf] in bohr, Cartesian powers
+ *     l/m/n[nbf] in get_cartesian_shell order (src/basis.f90:371-409),
+ *     coef[n2] (normalised, src/basis.f90:486-487) and alpha[n2];
+ *   - matrices are n x n, FP64, column-major (S[i + j*nbf], src/qc.c:15),
+ *     symmetric, full storage;
+ *   - every call is synchronous: outputs are valid on return;
+ *   - status: 0 on success, non-zero on error; hfx_last_error() explains.
+ * The caller owns every input and output buffer; the handle owns its device
+ * memory (basis, shell-pair data, screened pair lists, Rys tables) until
+ * hfx_destroy. One handle per host thread; calls on one handle are serial.
+ */
+#ifndef HFX_H
+#define HFX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hfx_engine hfx_t;
+
+typedef struct {
+    double schwarz_threshold; /* drop quartets with Q_ab * Q_cd below this (default 1e-12) */
+    double fp32_threshold;    /* quartets with Q_ab * Q_cd below this use FP32 ERI arithmetic;
+                                 0 = all FP64 (default 0) */
+    int device;               /* HIP device ordinal (default 0) */
+    int verbose;              /* print set-up statistics to stderr (default 0) */
+} hfx_opts;
+
+typedef struct {
+    double seconds;           /* wall time of the call, device-synchronised */
+    double quartets_fp64;     /* shell quartets evaluated in FP64 */
+    double quartets_fp32;     /* shell quartets evaluated in FP32 */
+    double flops_fp64;        /* algorithmic FLOPs of the FP64 part (frozen per-class model) */
+    double flops_fp32;        /* algorithmic FLOPs of the FP32 part */
+    double jk_seconds;        /* J/K kernel time, HIP events on the engine's stream */
+} hfx_stats;
+
+/* Fill *o with the defaults. Returns 0. */
+int hfx_default_opts(hfx_opts *o);
+
+/*
+ * Create an engine for one geometry.
+ * Replaces the set-up half of getints2/getInts (src/gaussians_hfsolver.f90:13-34,
+ * src/qc.h:35-39): the same per-basis-function arrays, regrouped into shells
+ * inside (as src/gaussians_libint.f90:59-122 does) and screened by Schwarz
+ * bounds computed on the GPU. nat/atno/xyz (xyz[3*nat], bohr, atom-major as
+ * the Fortran xyz(3,nat)) give the nuclei: V, forces and hfx_set_geometry
+ * need them; every basis-function centre must coincide with one atom.
+ */
+int hfx_create(int nbf, const int *nprim, const int *istart, const double *x, const double *y, const double *z,
+               const int *l, const int *m, const int *n, int n2, const double *coef, const double *alpha, int nat,
+               const int *atno, const double *xyz, const hfx_opts *o, hfx_t **h);
+
+/*
+ * Two-electron Fock matrix of a density, host buffers.
+ * Replaces G = formG(P, TT) (src/scf.f90:26-37, :109):
+ *   J_ij = sum_kl P_kl (ij|kl),  K_ij = sum_kl P_kl (il|kj),  G = J - K/2.
+ * P: n x n input (symmetrised as (P + P^T)/2). G, J, K: n x n outputs; any of
+ * them may be NULL. stats may be NULL.
+ */
+int hfx_fock_2e(hfx_t *h, const double *P, double *G, double *J, double *K, hfx_stats *stats);
+
+/*
+ * Same on device buffers (HIP device pointers of n*n doubles, resident in
+ * HBM): no host transfers. Used by benchmarks and by callers that keep the
+ * density on the GPU. Synchronous.
+ */
+int hfx_fock_2e_device(hfx_t *h, const double *dP, double *dG, double *dJ, double *dK, hfx_stats *stats);
+
+/*
+ * One-electron matrices S, T, V (n x n, any may be NULL) for the current
+ * geometry. Counterpart of getS/getT/getV (src/qc.c:6-93, src/qc.h:13-33);
+ * V includes the nuclear charges atno. Evaluated on the host (they are O(n^2)).
+ */
+int hfx_one_electron(hfx_t *h, double *S, double *T, double *V);
+
+/*
+ * Two-electron part of the RHF nuclear gradient (no reference counterpart):
+ *   g_X = sum_{ijkl} [1/2 P_ij P_kl - 1/4 P_ik P_jl] d(ij|kl)/dX,
+ * evaluated with derivative integrals on the GPU. g: 3*nat (x,y,z per atom).
+ */
+int hfx_grad_2e(hfx_t *h, const double *P, double *g);
+
+/*
+ * One-electron part of the RHF gradient:
+ *   g_X = sum_ij P_ij dH_ij/dX - sum_ij W_ij dS_ij/dX  (Hellmann-Feynman term included),
+ * W = energy-weighted density. g: 3*nat. Evaluated on the host.
+ */
+int hfx_grad_1e(hfx_t *h, const double *P, const double *W, double *g);
+
+/* Move the nuclei (xyz[3*nat], bohr); basis functions follow their atom.
+ * Rebuilds pair data, Schwarz bounds and quartet lists. */
+int hfx_set_geometry(hfx_t *h, const double *xyz);
+
+/*
+ * Integrals of explicit shell quartets (testing and store-mode use).
+ * shells[4*k .. 4*k+3] = (A, B, C, D) shell indices (shells numbered in
+ * basis-function order); out receives the blocks one after another, each as
+ * out[((a*nB + b)*nC + c)*nD + d] with a, b, c, d the Cartesian components of
+ * A, B, C, D in reference order (d fastest).
+ * Evaluated on the GPU in FP64 with no screening.
+ */
+int hfx_eri_shells(hfx_t *h, int nq, const int *shells, double *out);
+
+/* Shell structure: number of shells; per shell its first basis function,
+ * angular momentum and atom (arrays of length nshell, may be NULL). */
+int hfx_shells(const hfx_t *h, int *nshell, int *first_bf, int *L, int *atom);
+
+/* Statistics of the last set-up: number of significant shell pairs and of
+ * screened shell quartets in FP64 / FP32. */
+int hfx_screening(const hfx_t *h, double *npairs, double *nq_fp64, double *nq_fp32);
+
+const char *hfx_last_error(const hfx_t *h);
+void hfx_destroy(hfx_t *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFX_H */

@@ -288,6 +288,7 @@ def doscf(H, G_of_P, S, nocc, Nscf, tolE, tolP, alpha, P0=None, log=None):
                 r = SCFResult()
                 r.C, r.P, r.lam, r.E0, r.F, r.G, r.iterations, r.dP = C, P, lam, E0, F, G, it, dP
                 r.Pin = Pold
+                r.E_iter = list(Escf)
                 return r
         P = (1 - alpha) * Pold + alpha * P
     raise RuntimeError("SCF did not converge in %d iterations." % Nscf)

@@ -1,0 +1,57 @@
+"""Loader of the CPU check harness tests/csrc/liberi_check.so (the engine's
+Rys templates instantiated for the host). Test infrastructure only."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc", "eri_check.cpp")
+LIB = os.path.join(HERE, "csrc", "liberi_check.so")
+
+
+def build():
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", SRC, "-o", LIB], check=True)
+
+
+def load():
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+        build()
+    return ctypes.CDLL(LIB)
+
+
+def shells_of(B):
+    """Shells of a per-function basis (L, first function, centre, exponents,
+    shell coefficients = first component's, per-component scale)."""
+    out = []
+    i = 0
+    while i < B.nbf:
+        L = int(B.l[i] + B.m[i] + B.n[i])
+        nc = (L + 1) * (L + 2) // 2
+        k = int(B.nprim[i])
+        s = int(B.istart[i])
+        c0 = B.coef[s:s + k].copy()
+        sc = np.array([B.coef[B.istart[i + c]] / c0[0] for c in range(nc)])
+        out.append(dict(L=L, f0=i, cen=np.array([B.x[i], B.y[i], B.z[i]]), a=B.alpha[s:s + k].copy(), c=c0, sc=sc))
+        i += nc
+    return out
+
+
+def pair_class(la, lb):
+    return 0 if la == 0 else ((1 if lb == 0 else 2) if la == 1 else 3 + lb)
+
+
+def quartet(lib, A, B, C, D, fp32=False):
+    """Block (ab|cd) of shells oriented l_A >= l_B, l_C >= l_D, class(AB) >= class(CD)."""
+    nc = lambda s: (s["L"] + 1) * (s["L"] + 2) // 2
+    out = np.zeros(nc(A) * nc(B) * nc(C) * nc(D))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    args = [A["L"], B["L"], C["L"], D["L"]] + [p(s["cen"]) for s in (A, B, C, D)]
+    for s in (A, B, C, D):
+        args += [len(s["a"]), p(s["a"]), p(s["c"])]
+    rc = lib.chk_eri_quartet(*args, 1 if fp32 else 0, p(out))
+    if rc != 0:
+        raise ValueError("bad quartet orientation")
+    blk = out.reshape(nc(A), nc(B), nc(C), nc(D))
+    return blk * np.einsum("a,b,c,d->abcd", A["sc"], B["sc"], C["sc"], D["sc"])

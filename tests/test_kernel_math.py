@@ -1,0 +1,88 @@
+"""The engine's Rys integral templates (hfsolver_amd/csrc/eri_rys.h), compiled
+for the host, against the oracle's THO integrals (a restatement of
+src/cints.c:72-149): every class combination of the 6-31G** s/p/d basis,
+FP64 and FP32 arithmetic, plus the Rys tables against mpmath-free checks."""
+import ctypes
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from hfsolver_amd.clusters import water_cluster
+from tests import kernel_check as KC
+
+H2O = ([8, 1, 1], [[0.0, 0.0, 0.0], [1.809, 0.0, 0.0], [-0.453549, 1.751221, 0.0]])
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return KC.load()
+
+
+def _oriented(A, B, C, D):
+    if A["L"] < B["L"]:
+        A, B = B, A
+    if C["L"] < D["L"]:
+        C, D = D, C
+    if KC.pair_class(A["L"], B["L"]) < KC.pair_class(C["L"], D["L"]):
+        A, B, C, D = C, D, A, B
+    return A, B, C, D
+
+
+def _ref_block(Bas, A, B, C, D):
+    nc = lambda s: (s["L"] + 1) * (s["L"] + 2) // 2
+    ref = np.zeros((nc(A), nc(B), nc(C), nc(D)))
+    for a, b, c, d in itertools.product(*(range(nc(s)) for s in (A, B, C, D))):
+        ref[a, b, c, d] = O.eri(Bas, A["f0"] + a, B["f0"] + b, C["f0"] + c, D["f0"] + d)
+    return ref
+
+
+def test_all_class_combinations_fp64(lib):
+    Bas = O.Basis(*H2O)
+    sh = KC.shells_of(Bas)
+    seen = {}
+    for (i, j, k, l) in itertools.product(range(len(sh)), repeat=4):
+        A, B, C, D = sh[i], sh[j], sh[k], sh[l]
+        if not (A["L"] >= B["L"] and C["L"] >= D["L"] and
+                KC.pair_class(A["L"], B["L"]) >= KC.pair_class(C["L"], D["L"])):
+            continue
+        key = (A["L"], B["L"], C["L"], D["L"])
+        if seen.get(key, 0) >= 2:
+            continue
+        seen[key] = seen.get(key, 0) + 1
+        v = KC.quartet(lib, A, B, C, D)
+        ref = _ref_block(Bas, A, B, C, D)
+        assert np.abs(v - ref).max() < 5e-15 * max(1.0, np.abs(ref).max()), key
+    assert len(seen) == 21
+
+
+def test_random_quartets_dimer_fp64_fp32(lib):
+    atno, xyz = water_cluster(2, seed=2)
+    Bas = O.Basis(atno, xyz)
+    sh = KC.shells_of(Bas)
+    rng = np.random.default_rng(7)
+    for _ in range(120):
+        A, B, C, D = _oriented(*[sh[i] for i in rng.integers(0, len(sh), 4)])
+        ref = _ref_block(Bas, A, B, C, D)
+        v64 = KC.quartet(lib, A, B, C, D)
+        v32 = KC.quartet(lib, A, B, C, D, fp32=True)
+        scale = np.abs(ref).max()
+        assert np.abs(v64 - ref).max() <= 1e-15 + 1e-14 * scale
+        # FP32 arithmetic: relative error bounded by a few thousand ulps of float
+        assert np.abs(v32 - ref).max() <= 1e-30 + 5e-6 * scale
+
+
+def test_rys_rule_reproduces_boys_moments(lib):
+    lib.chk_rys.restype = ctypes.c_double
+    u = np.zeros(5)
+    w = np.zeros(5)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    for nr in range(1, 6):
+        for T in [0.0, 1e-8, 0.37, 1.0, 4.5, 17.3, 33.0, 59.99, 60.0, 85.0, 1e3]:
+            lib.chk_rys(nr, ctypes.c_double(T), p(u), p(w))
+            F = O.boys(2 * nr - 1, T)
+            for k in range(2 * nr):
+                assert abs(np.sum(w[:nr] * u[:nr] ** k) - F[k]) <= 1e-15 * F[0] + 4e-15 * F[k]
+            assert np.all(np.diff(u[:nr]) > 0) and np.all(u[:nr] > 0) and np.all(u[:nr] < 1)

@@ -1,0 +1,150 @@
+"""Pins the oracle (oracle/) before it is trusted as the checker.
+
+- Boys function: closed forms of hfsolver's own test (src/tests/test_fm.f90:46-68).
+- Integrals, matrices, G(P), SCF: the compiled reference's outputs
+  (tests/golden/*.npz, written by tests/golden/make_golden.py).
+- Energies: the reference's literature known answers
+  (src/tests/test_gaussians_molecules.f90:106,110,183).
+- Packed 8-fold index: counts and permutation symmetry (src/tests/test_ijkl.f90).
+- Gradient: central finite differences of the oracle's own energy.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+H2O_TEST = ([8, 1, 1], [[0.0, 0.0, 0.0], [1.809, 0.0, 0.0], [-0.453549, 1.751221, 0.0]])
+CH4 = ([6, 1, 1, 1, 1], [[0.0, 0.0, 0.0], [2.05, 0.0, 0.0], [-0.683292, 1.932773, 0.0],
+                         [-0.683292, -0.966387, -1.673831], [-0.683292, -0.966387, 1.673831]])
+
+
+def _downwards(maxm, t):
+    F = np.zeros(maxm + 1)
+    for m in range(maxm - 1, -1, -1):
+        F[m] = (2 * t * F[m + 1] + math.exp(-t)) / (2 * m + 1)
+    return F
+
+
+def _asympt(maxm, t):
+    return np.array([math.exp(math.lgamma(m + 0.5) - (m + 0.5) * math.log(t)) / 2 for m in range(maxm + 1)])
+
+
+@pytest.mark.parametrize("t", [0.0, 1e-9, 1e-6, 0.1, 0.5, 2.0, 10.0, 15.0, 19.0, 20.0, 21.0, 25.0, 30.0, 50.0,
+                               100.0, 150.0, 200.0, 500.0, 2000.0, 20000.0, 2e5, 2e6, 2e8])
+def test_boys_known_answers(t):
+    ref = _downwards(1000, t) if t < 510 else _asympt(1000, t)
+    F0 = 1.0 if t < np.finfo(float).eps else 0.5 * math.sqrt(math.pi / t) * math.erf(math.sqrt(t))
+    assert abs(ref[0] - F0) < 1e-15
+    for m in (0, 1, 2, 5, 9, 17, 40, 120, 500):
+        G = O.boys(m, t)
+        assert np.all(np.abs(G - ref[:m + 1]) < 1e-15)
+        rel = np.abs(G - ref[:m + 1]) / np.maximum(np.maximum(np.abs(G), np.abs(ref[:m + 1])), 1e-300)
+        assert np.all((np.abs(G - ref[:m + 1]) < np.finfo(float).tiny) | (rel < 1e-12))
+
+
+def test_boys_domain():
+    with pytest.raises(ValueError):
+        O.boys(501, 1.0)
+    with pytest.raises(ValueError):
+        O.boys(3, -1.0)
+
+
+def test_intindex_symmetry_and_count():
+    n = 6
+    seen = set()
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for l in range(n):
+                    v = O.intindex(i, j, k, l)
+                    for (a, b, c, d) in [(j, i, k, l), (i, j, l, k), (j, i, l, k), (k, l, i, j), (l, k, i, j),
+                                         (k, l, j, i), (l, k, j, i)]:
+                        assert O.intindex(a, b, c, d) == v
+                    seen.add(v)
+    assert len(seen) == n * (n ** 3 + 2 * n ** 2 + 3 * n + 2) // 8
+    assert max(seen) == len(seen) - 1
+
+
+@pytest.mark.parametrize("name", ["h2o_testgeom", "h2o_config0", "ch4", "water2_seed2"])
+def test_one_electron_matches_reference(golden, name):
+    g = golden(name)
+    B = O.Basis(g["atno"], g["xyz"])
+    S, T, V = O.one_electron(B)
+    assert np.abs(S - g["S"]).max() < 1e-14
+    assert np.abs(T - g["T"]).max() < 1e-13
+    assert np.abs(V - g["V"]).max() < 1e-13
+    assert np.allclose(np.diag(S), 1.0, atol=2e-7)   # FP32 norm_prim factor: not exactly 1
+
+
+@pytest.mark.parametrize("name", ["h2o_config0", "ch4", "water2_seed2"])
+def test_eri_matches_reference_int2(golden, name):
+    g = golden(name)
+    B = O.Basis(g["atno"], g["xyz"])
+    idx, val = g["int2_idx"], g["int2_val"]
+    assert len(val) > 50
+    for (i, j, k, l), v in zip(idx[::3], val[::3]):
+        assert abs(O.eri(B, i - 1, j - 1, k - 1, l - 1) - v) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["h2o_testgeom", "h2o_config0", "water2_seed2"])
+def test_formG_matches_reference(golden, name):
+    g = golden(name)
+    B = O.Basis(g["atno"], g["xyz"])
+    J, K = O.jk(B, g["P"])
+    assert np.abs(J - g["J"]).max() < 1e-12
+    assert np.abs(K - g["K"]).max() < 1e-12
+    assert np.abs((J - 0.5 * K) - g["G"]).max() < 1e-12
+
+
+def test_packed_int2_matches_formG(golden):
+    g = golden("h2o_config0")
+    B = O.Basis(g["atno"], g["xyz"])
+    ints = O.get_ints(B)
+    n = B.nbf
+    P = g["P"]
+    J = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            J[i, j] = sum(P[k, l] * ints[O.intindex(i, j, k, l)] for k in range(n) for l in range(n))
+    assert np.abs(J - g["J"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["h2o_testgeom", "h2o_config0"])
+def test_scf_matches_reference(golden, name):
+    g = golden(name)
+    r = O.rhf_gauss(g["atno"], g["xyz"], nthreads=0)
+    assert abs(r.Etot - g["Etot"]) < 1e-9
+    assert abs(r.E0 - g["E0"]) < 1e-9
+    assert np.abs(r.lam - g["lam"]).max() < 1e-9
+    assert abs(r.Ekin - g["Ekin"]) < 1e-8
+    assert r.iterations == len(g["E_iter"])
+    # per-iteration E0 as printed by doscf (src/scf.f90:114), %.15g-ish text
+    assert np.abs(np.array(r.E_iter) - g["E_iter"]).max() < 1e-9
+
+
+def test_literature_energies():
+    # src/tests/test_gaussians_molecules.f90:183 (H2O) and :106,110 (CH4)
+    r = O.rhf_gauss(*H2O_TEST)
+    assert abs(r.Etot - (-76.02315869)) < 1e-8
+    r = O.rhf_gauss(*CH4)
+    assert abs(r.Etot - (-40.20170036)) < 1e-8
+    assert abs(r.lam[4] - (-0.54451008)) < 1e-8
+
+
+def test_gradient_finite_differences():
+    atno, xyz = H2O_TEST
+    xyz = np.array(xyz, dtype=float)
+    r = O.rhf_gauss(atno, xyz)
+    g = O.gradient(r)
+    assert np.abs(g.sum(axis=0)).max() < 1e-8          # translational invariance
+    h = 1e-4
+    for (a, d) in [(0, 0), (2, 1)]:
+        e = []
+        for s in (+1, -1):
+            x = xyz.copy()
+            x[a, d] += s * h
+            e.append(O.rhf_gauss(atno, x, tolP=1e-10, tolE=1e-12, Nscf=80).Etot)
+        fd = (e[0] - e[1]) / (2 * h)
+        assert abs(fd - g[a, d]) < 1e-6
