@@ -39,6 +39,7 @@ def main():
     ap.add_argument("--density", choices=["core", "scf"], default="core")
     ap.add_argument("--fp32-threshold", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path runs on a single MI355X "
@@ -57,7 +58,7 @@ def main():
     n = b.nbf
     log("bench: %d waters, n = %d" % (args.nwater, n))
     t = time.time()
-    e64 = Engine(b, schwarz_threshold=1e-12)
+    e64 = Engine(b, schwarz_threshold=1e-12, verbose=args.verbose)
     log("  FP64 engine set-up %.2f s; screening %s" % (time.time() - t, e64.screening()))
     t = time.time()
     emx = Engine(b, schwarz_threshold=1e-12, fp32_threshold=args.fp32_threshold)

@@ -170,19 +170,23 @@ HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, c
     }
     const R abx = R(bra.ABx), aby = R(bra.ABy), abz = R(bra.ABz);
     const R cdx = R(ket.ABx), cdy = R(ket.ABy), cdz = R(ket.ABz);
-    for (int ip = 0; ip < bra.np; ip++) {
-        const PrimPair P = prims[bra.p0 + ip];
-        for (int kp = 0; kp < ket.np; kp++) {
-            const PrimPair Qp = prims[ket.p0 + kp];
+    for (int i1 = 0; i1 < bra.np; i1++) {
+        const PrimPair P = prims[bra.p0 + i1];
+        for (int k1 = 0; k1 < ket.np; k1++) {
+            const PrimPair Qp = prims[ket.p0 + k1];
             const double p = P.p, q = Qp.p, pq = p + q;
-            const double ipq = 1.0 / pq;
+            const double ip = P.ip, iq = Qp.ip;
+            const double isq = hfx_rsqrt(pq);
+            const double ipq = isq * isq;
             const double rho = p * q * ipq;
             const double PQx = P.Px - Qp.Px, PQy = P.Py - Qp.Py, PQz = P.Pz - Qp.Pz;
             const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = HFX_TWO_PI_52 * P.K * Qp.K / (p * q * sqrt(pq));
+            const double pref = HFX_TWO_PI_52 * P.K * Qp.K * ip * iq * isq;
             double u[NR], w[NR];
-            rys_roots<NR>(T, u, w, tab);
-            const double ip = 1.0 / p, iq = 1.0 / q;
+            if constexpr (NR <= 2)
+                rys_roots_small<NR>(T, u, w, tab.boys);
+            else
+                rys_roots<NR>(T, u, w, tab);
             // the ket PrimPair stores Q - C in its PA fields
 #pragma unroll
             for (int r = 0; r < NR; r++) {
@@ -227,18 +231,51 @@ HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, c
     }
 }
 
-// J/K digestion of one slice of a quartet, accumulated in registers and
-// flushed through `add(ptr, value)` (atomics on the device, plain adds on the
-// host). Buffers J, K are n x n (row index = first function); the caller
-// symmetrises J = Jb + Jb^T, K = Kb + Kb^T afterwards. s = quartet degeneracy
-// factor (1/2 for A==B, for C==D and for bra pair == ket pair).
+// J/K digestion of one slice of a quartet: the contributions of one quartet
+// to the six density-contracted blocks, accumulated in registers. The caller
+// scales them by the quartet degeneracy factor s (1/2 for A==B, for C==D and
+// for bra pair == ket pair) and flushes them into unsymmetrised buffers Jb, Kb
+// (row index = first function), symmetrised afterwards (J = Jb + Jb^T, K = Kb + Kb^T):
 //   Jb_ab += 2 s sum_cd (ab|cd) D_cd,  Jb_cd += 2 s sum_ab (ab|cd) D_ab,
 //   Kb_ac += s (ab|cd) D_bd,  Kb_ad += s (ab|cd) D_bc,
 //   Kb_bc += s (ab|cd) D_ad,  Kb_bd += s (ab|cd) D_ac.
-template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R, typename Add>
-HFX_HD HFX_INLINE void digest_jk(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], int fa, int fb, int fc,
-                                 int fd, const double* __restrict__ D, int n, double s, double* J, double* K,
-                                 Add add) {
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+struct JKAcc {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    R Jab[Q::NAt][Q::NB], Jcd[Q::NCt][Q::ND];
+    R Kac[Q::NAt][Q::NCt], Kad[Q::NAt][Q::ND], Kbc[Q::NB][Q::NCt], Kbd[Q::NB][Q::ND];
+};
+
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void jk_zero(JKAcc<LA, LB, LC, LD, SPA, SPC, R>& acc) {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+#pragma unroll
+    for (int a = 0; a < Q::NAt; a++) {
+#pragma unroll
+        for (int b = 0; b < Q::NB; b++) acc.Jab[a][b] = R(0);
+#pragma unroll
+        for (int c = 0; c < Q::NCt; c++) acc.Kac[a][c] = R(0);
+#pragma unroll
+        for (int d = 0; d < Q::ND; d++) acc.Kad[a][d] = R(0);
+    }
+#pragma unroll
+    for (int b = 0; b < Q::NB; b++) {
+#pragma unroll
+        for (int c = 0; c < Q::NCt; c++) acc.Kbc[b][c] = R(0);
+#pragma unroll
+        for (int d = 0; d < Q::ND; d++) acc.Kbd[b][d] = R(0);
+    }
+#pragma unroll
+    for (int c = 0; c < Q::NCt; c++)
+#pragma unroll
+        for (int d = 0; d < Q::ND; d++) acc.Jcd[c][d] = R(0);
+}
+
+// fa/fc: first function of the thread's a / c slice (fa + sa when slicing).
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void jk_accumulate(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], int fa, int fb, int fc,
+                                     int fd, const double* __restrict__ D, int n,
+                                     JKAcc<LA, LB, LC, LD, SPA, SPC, R>& acc) {
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
     R Dab[NAt][NB], Dcd[NCt][ND], Dbd[NB][ND], Dbc[NB][NCt], Dad[NAt][ND], Dac[NAt][NCt];
@@ -262,29 +299,6 @@ HFX_HD HFX_INLINE void digest_jk(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>
     for (int c = 0; c < NCt; c++)
 #pragma unroll
         for (int d = 0; d < ND; d++) Dcd[c][d] = R(D[(size_t)(fc + c) * n + fd + d]);
-
-    R Jab[NAt][NB], Jcd[NCt][ND], Kac[NAt][NCt], Kad[NAt][ND], Kbc[NB][NCt], Kbd[NB][ND];
-#pragma unroll
-    for (int a = 0; a < NAt; a++) {
-#pragma unroll
-        for (int b = 0; b < NB; b++) Jab[a][b] = R(0);
-#pragma unroll
-        for (int c = 0; c < NCt; c++) Kac[a][c] = R(0);
-#pragma unroll
-        for (int d = 0; d < ND; d++) Kad[a][d] = R(0);
-    }
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-#pragma unroll
-        for (int c = 0; c < NCt; c++) Kbc[b][c] = R(0);
-#pragma unroll
-        for (int d = 0; d < ND; d++) Kbd[b][d] = R(0);
-    }
-#pragma unroll
-    for (int c = 0; c < NCt; c++)
-#pragma unroll
-        for (int d = 0; d < ND; d++) Jcd[c][d] = R(0);
-
 #pragma unroll
     for (int a = 0; a < NAt; a++)
 #pragma unroll
@@ -294,32 +308,11 @@ HFX_HD HFX_INLINE void digest_jk(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>
 #pragma unroll
                 for (int d = 0; d < ND; d++) {
                     const R v = eri[((a * NB + b) * NCt + c) * ND + d];
-                    Jab[a][b] = v * Dcd[c][d] + Jab[a][b];
-                    Jcd[c][d] = v * Dab[a][b] + Jcd[c][d];
-                    Kac[a][c] = v * Dbd[b][d] + Kac[a][c];
-                    Kad[a][d] = v * Dbc[b][c] + Kad[a][d];
-                    Kbc[b][c] = v * Dad[a][d] + Kbc[b][c];
-                    Kbd[b][d] = v * Dac[a][c] + Kbd[b][d];
+                    acc.Jab[a][b] = v * Dcd[c][d] + acc.Jab[a][b];
+                    acc.Jcd[c][d] = v * Dab[a][b] + acc.Jcd[c][d];
+                    acc.Kac[a][c] = v * Dbd[b][d] + acc.Kac[a][c];
+                    acc.Kad[a][d] = v * Dbc[b][c] + acc.Kad[a][d];
+                    acc.Kbc[b][c] = v * Dad[a][d] + acc.Kbc[b][c];
+                    acc.Kbd[b][d] = v * Dac[a][c] + acc.Kbd[b][d];
                 }
-    const double s2 = 2.0 * s;
-#pragma unroll
-    for (int a = 0; a < NAt; a++) {
-#pragma unroll
-        for (int b = 0; b < NB; b++) add(&J[(size_t)(fa + a) * n + fb + b], s2 * double(Jab[a][b]));
-#pragma unroll
-        for (int c = 0; c < NCt; c++) add(&K[(size_t)(fa + a) * n + fc + c], s * double(Kac[a][c]));
-#pragma unroll
-        for (int d = 0; d < ND; d++) add(&K[(size_t)(fa + a) * n + fd + d], s * double(Kad[a][d]));
-    }
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-#pragma unroll
-        for (int c = 0; c < NCt; c++) add(&K[(size_t)(fb + b) * n + fc + c], s * double(Kbc[b][c]));
-#pragma unroll
-        for (int d = 0; d < ND; d++) add(&K[(size_t)(fb + b) * n + fd + d], s * double(Kbd[b][d]));
-    }
-#pragma unroll
-    for (int c = 0; c < NCt; c++)
-#pragma unroll
-        for (int d = 0; d < ND; d++) add(&J[(size_t)(fc + c) * n + fd + d], s2 * double(Jcd[c][d]));
 }

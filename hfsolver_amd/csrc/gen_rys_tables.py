@@ -45,6 +45,15 @@ NINT = 60         # intervals on [0, TMAX); exp(-T) T^(m-1/2)/Gamma(m+1/2) < 1e-
 TMAX = H * NINT
 DEG = 13          # Chebyshev degree per interval
 DPS = 60
+# Boys-function Taylor table for the one- and two-root rules (closed form from
+# F_0..F_3): F_m(T) = sum_{k<BOYS_K} F_{m+k}(T0) (T0-T)^k / k!, T0 on a grid of
+# step BOYS_DT, |T - T0| <= BOYS_DT/2, error < (1/16)^9/9! ~ 4e-17 relative.
+BOYS_DT = 0.125
+BOYS_TMAX = 36.0
+BOYS_K = 9
+BOYS_MMAX = 3                      # highest order needed (two roots: F_0..F_3)
+BOYS_NCOL = BOYS_MMAX + BOYS_K     # F_0 .. F_11 per grid point
+BOYS_NPT = int(BOYS_TMAX / BOYS_DT) + 1
 
 mp.mp.dps = DPS
 
@@ -137,7 +146,14 @@ def main():
            "#define RYS_DEG %d" % DEG,
            "#define RYS_H %.17g" % H,
            "#define RYS_TMAX %.17g" % TMAX,
-           "#define RYS_CHEB_SIZE %d" % o]
+           "#define RYS_CHEB_SIZE %d" % o,
+           "#define BOYS_DT %.17g" % BOYS_DT,
+           "#define BOYS_INV_DT %.17g" % (1 / BOYS_DT),
+           "#define BOYS_TMAX %.17g" % BOYS_TMAX,
+           "#define BOYS_K %d" % BOYS_K,
+           "#define BOYS_NCOL %d" % BOYS_NCOL,
+           "#define BOYS_NPT %d" % BOYS_NPT,
+           "#define BOYS_SIZE %d" % (BOYS_NCOL * BOYS_NPT)]
     with open(os.path.join(os.path.dirname(out), "rys_params.h"), "w") as f:
         f.write("\n".join(hdr) + "\n")
     lines = ["// Generated by gen_rys_tables.py - do not edit. Data of rys_params.h;",
@@ -162,6 +178,12 @@ def main():
     lines.append("RYS_TABLE_QUALIFIER double RYS_HERM_W[RYS_NMAX * RYS_NMAX] = {")
     for row in hw:
         lines.append("  " + ", ".join("%.17e" % v for v in row) + ",")
+    lines.append("};")
+    # Boys table: BOYS_TAB[i * BOYS_NCOL + m] = F_m(i * BOYS_DT)
+    lines.append("RYS_TABLE_QUALIFIER double BOYS_TAB[BOYS_SIZE] = {")
+    for i in range(BOYS_NPT):
+        fm = boys_moments(mp.mpf(i) * mp.mpf(BOYS_DT), BOYS_NCOL - 1)
+        lines.append("  " + ", ".join("%.17e" % float(v) for v in fm) + ",")
     lines.append("};")
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -39,22 +39,26 @@ HFX_HD constexpr int pair_class(int la, int lb) {
 HFX_HD constexpr int pair_class_la(int c) { return c == 0 ? 0 : (c <= 2 ? 1 : 2); }
 HFX_HD constexpr int pair_class_lb(int c) { return c == 0 ? 0 : (c == 1 ? 0 : (c == 2 ? 1 : c - 3)); }
 
-// One shell pair (A, B) with l_A >= l_B. 64 bytes.
+// One shell pair (A, B) with l_A >= l_B. 80 bytes.
 struct ShellPair {
     int A, B;      // shell indices
     int fa, fb;    // first (scaled-basis) function of A and of B
     int p0, np;    // primitive pairs [p0, p0 + np)
-    int cls, pad;  // pair class
+    int cls;       // pair class
+    int cpa, cpb;  // first function of A / B among the functions of shells of the same L
+    int pad[3];
     double ABx, ABy, ABz;  // A - B
     double Q;              // Schwarz bound sqrt(max |(ab|ab)|)
 };
 
-// One primitive pair of a shell pair. 64 bytes.
+// One primitive pair of a shell pair. 80 bytes.
 struct PrimPair {
     double p;             // alpha_a + alpha_b
+    double ip;            // 1 / p
     double Px, Py, Pz;    // (alpha_a A + alpha_b B) / p
     double PAx, PAy, PAz; // P - A
     double K;             // c_a c_b exp(-alpha_a alpha_b |A-B|^2 / p), shell-level coefficients
+    double pad;
 };
 
 // Exponents of a primitive pair (used by the derivative integrals).

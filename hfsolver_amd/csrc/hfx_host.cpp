@@ -44,6 +44,7 @@ struct Shell {
     int f0;       // first basis function
     int nprim;
     int prim0;    // into alpha / coef
+    int cpos;     // first function among the functions of shells of the same L
 };
 
 struct DevBuf {
@@ -67,6 +68,9 @@ struct hfx_engine {
     std::vector<double> alpha, coef;  // shell-level primitives
     std::vector<double> scale;        // per basis function
     std::vector<int> bf_atom;
+    std::vector<int> bfL[HFX_LMAX + 1];   // functions of the shells of each L, in shell order
+    int* d_bfL[HFX_LMAX + 1] = {nullptr};
+    size_t lds_budget = 80 * 1024;
     // pairs
     std::vector<ShellPair> pairs;
     std::vector<PrimPair> prims;
@@ -74,6 +78,9 @@ struct hfx_engine {
     std::vector<int> cls_list[HFX_NPAIRCLASS];  // significant pairs, Q descending
     struct Combo {
         int cb, ck;
+        int* d_ket = nullptr;   // ket sublist (equal primitive-pair count)
+        int nket = 0;
+        int npk = 0;            // its primitive-pair count
         int ntiles[2] = {0, 0};
         int* d_ti0[2] = {nullptr, nullptr};
         int* d_tj0[2] = {nullptr, nullptr};
@@ -81,6 +88,14 @@ struct hfx_engine {
         int* d_kend[2] = {nullptr, nullptr};
         double nq[2] = {0, 0};
         double nprimq[2] = {0, 0};
+        // row mode (K rows of the bra shells accumulated in LDS)
+        bool rowmode = false;
+        int boys_lds = 0;
+        int ncols = 0, offd = 0;
+        int nblk[2] = {0, 0};
+        int chunk[2] = {0, 0};
+        int* d_blk_row[2] = {nullptr, nullptr};
+        int* d_blk_k0[2] = {nullptr, nullptr};
     };
     std::vector<Combo> combos;
     double npairs_sig = 0;
@@ -148,7 +163,8 @@ void free_geometry(hfx_t* h) {
 }
 
 RysTables host_tables() {
-    return RysTables{hfx_host_tables::RYS_CHEB, hfx_host_tables::RYS_HERM_U, hfx_host_tables::RYS_HERM_W};
+    return RysTables{hfx_host_tables::RYS_CHEB, hfx_host_tables::RYS_HERM_U, hfx_host_tables::RYS_HERM_W,
+                     hfx_host_tables::BOYS_TAB};
 }
 
 void rys_roots_rt(int nr, double T, double* u, double* w) {
@@ -225,6 +241,11 @@ int build_shells(hfx_t* h, int nbf, const int* nprim, const int* istart, const d
         i += nc;
     }
     h->nbf = nbf;
+    for (int L = 0; L <= HFX_LMAX; L++) h->bfL[L].clear();
+    for (Shell& sh : h->shells) {
+        sh.cpos = (int)h->bfL[sh.L].size();
+        for (int c = 0; c < ncart(sh.L); c++) h->bfL[sh.L].push_back(sh.f0 + c);
+    }
     return 0;
 }
 
@@ -248,7 +269,9 @@ void build_pairs(hfx_t* h) {
             sp.fa = sa.f0;
             sp.fb = sb.f0;
             sp.cls = pair_class(sa.L, sb.L);
-            sp.pad = 0;
+            sp.cpa = h->shells[A].cpos;
+            sp.cpb = h->shells[B].cpos;
+            sp.pad[0] = sp.pad[1] = sp.pad[2] = 0;
             sp.ABx = ra[0] - rb[0];
             sp.ABy = ra[1] - rb[1];
             sp.ABz = ra[2] - rb[2];
@@ -261,6 +284,8 @@ void build_pairs(hfx_t* h) {
                     const double p = a + b;
                     PrimPair pp;
                     pp.p = p;
+                    pp.ip = 1.0 / p;
+                    pp.pad = 0.0;
                     pp.Px = (a * ra[0] + b * rb[0]) / p;
                     pp.Py = (a * ra[1] + b * rb[1]) / p;
                     pp.Pz = (a * ra[2] + b * rb[2]) / p;
@@ -351,69 +376,126 @@ int build_lists(hfx_t* h) {
             if (upload(h, &h->d_cls_list[c], h->cls_list[c], true)) return 1;
     }
     h->combos.clear();
+    // position of every significant pair in its class list
+    std::vector<int> pos(h->pairs.size(), -1);
+    for (int c = 0; c < HFX_NPAIRCLASS; c++)
+        for (int i = 0; i < (int)h->cls_list[c].size(); i++) pos[h->cls_list[c][i]] = i;
     for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
         for (int ck = 0; ck <= cb; ck++) {
             const std::vector<int>& bl = h->cls_list[cb];
-            const std::vector<int>& kl = h->cls_list[ck];
-            if (bl.empty() || kl.empty()) continue;
-            const int nb = (int)bl.size();
-            std::vector<double> kq(kl.size()), knp(kl.size() + 1, 0.0);
-            for (size_t j = 0; j < kl.size(); j++) {
-                kq[j] = h->pairs[kl[j]].Q;
-                knp[j + 1] = knp[j] + h->pairs[kl[j]].np;
-            }
-            // number of kets with Q_i Q_j >= t: prefix of the descending list
-            auto prefix = [&](double qi, double t) -> int {
-                if (qi <= 0) return 0;
-                const double need = t / qi;
-                return (int)(std::upper_bound(kq.begin(), kq.end(), need, [](double v, double e) { return v > e; }) -
-                             kq.begin());
-            };
-            std::vector<int> kmax(nb), k64(nb);
-            for (int i = 0; i < nb; i++) {
-                const double qi = h->pairs[bl[i]].Q;
-                int km = prefix(qi, thr);
-                if (cb == ck) km = std::min(km, i + 1);
-                int k6 = thr32 > 0 ? prefix(qi, thr32) : km;
-                k64[i] = std::min(k6, km);
-                kmax[i] = km;
-            }
-            hfx_engine::Combo cmb;
-            cmb.cb = cb;
-            cmb.ck = ck;
-            FlopModel fm = flop_model(pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck));
-            (void)fm;
-            for (int prec = 0; prec < 2; prec++) {
-                std::vector<int> kb(nb), ke(nb), ti0, tj0;
-                for (int i = 0; i < nb; i++) {
-                    kb[i] = prec == 0 ? 0 : k64[i];
-                    ke[i] = prec == 0 ? k64[i] : kmax[i];
-                    const double nqi = std::max(0, ke[i] - kb[i]);
-                    cmb.nq[prec] += nqi;
-                    if (nqi > 0) cmb.nprimq[prec] += h->pairs[bl[i]].np * (knp[ke[i]] - knp[kb[i]]);
+            if (bl.empty() || h->cls_list[ck].empty()) continue;
+            // ket sublists of equal primitive-pair count, each still Q-descending
+            std::vector<int> nps;
+            for (int pid : h->cls_list[ck]) nps.push_back(h->pairs[pid].np);
+            std::sort(nps.begin(), nps.end());
+            nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
+            for (int npk : nps) {
+                std::vector<int> kl;
+                for (int pid : h->cls_list[ck])
+                    if (h->pairs[pid].np == npk) kl.push_back(pid);
+                const int nb = (int)bl.size();
+                const int nk = (int)kl.size();
+                std::vector<double> kq(nk);
+                std::vector<int> kpos(nk);
+                for (int j = 0; j < nk; j++) {
+                    kq[j] = h->pairs[kl[j]].Q;
+                    kpos[j] = pos[kl[j]];
                 }
-                for (int r0 = 0; r0 < nb; r0 += HFX_TILE_ROWS) {
-                    int lo = 1 << 30, hi = 0;
-                    for (int r = r0; r < std::min(nb, r0 + HFX_TILE_ROWS); r++)
-                        if (ke[r] > kb[r]) {
-                            lo = std::min(lo, kb[r]);
-                            hi = std::max(hi, ke[r]);
+                // number of kets with Q_i Q_j >= t: prefix of the descending list
+                auto prefix = [&](double qi, double t) -> int {
+                    if (qi <= 0) return 0;
+                    const double need = t / qi;
+                    return (int)(std::upper_bound(kq.begin(), kq.end(), need,
+                                                  [](double v, double e) { return v > e; }) - kq.begin());
+                };
+                std::vector<int> kmax(nb), k64(nb);
+                for (int i = 0; i < nb; i++) {
+                    const double qi = h->pairs[bl[i]].Q;
+                    int km = prefix(qi, thr);
+                    // same class: each unordered pair of pairs once, ket position <= bra position
+                    if (cb == ck) km = std::min(km, (int)(std::upper_bound(kpos.begin(), kpos.end(), i) - kpos.begin()));
+                    const int k6 = thr32 > 0 ? prefix(qi, thr32) : km;
+                    k64[i] = std::min(k6, km);
+                    kmax[i] = km;
+                }
+                hfx_engine::Combo cmb;
+                cmb.cb = cb;
+                cmb.ck = ck;
+                cmb.nket = nk;
+                cmb.npk = npk;
+                if (upload(h, &cmb.d_ket, kl, true)) return 1;
+                for (int prec = 0; prec < 2; prec++) {
+                    std::vector<int> kb(nb), ke(nb), ti0, tj0;
+                    for (int i = 0; i < nb; i++) {
+                        kb[i] = prec == 0 ? 0 : k64[i];
+                        ke[i] = prec == 0 ? k64[i] : kmax[i];
+                        const double nqi = std::max(0, ke[i] - kb[i]);
+                        cmb.nq[prec] += nqi;
+                        cmb.nprimq[prec] += nqi * h->pairs[bl[i]].np * npk;
+                    }
+                    for (int r0 = 0; r0 < nb; r0 += HFX_TILE_ROWS) {
+                        int lo = 1 << 30, hi = 0;
+                        for (int r = r0; r < std::min(nb, r0 + HFX_TILE_ROWS); r++)
+                            if (ke[r] > kb[r]) {
+                                lo = std::min(lo, kb[r]);
+                                hi = std::max(hi, ke[r]);
+                            }
+                        if (hi <= lo) continue;
+                        for (int j0 = (lo / HFX_TILE_COLS) * HFX_TILE_COLS; j0 < hi; j0 += HFX_TILE_COLS) {
+                            ti0.push_back(r0);
+                            tj0.push_back(j0);
                         }
-                    if (hi <= lo) continue;
-                    for (int j0 = (lo / HFX_TILE_COLS) * HFX_TILE_COLS; j0 < hi; j0 += HFX_TILE_COLS) {
-                        ti0.push_back(r0);
-                        tj0.push_back(j0);
+                    }
+                    cmb.ntiles[prec] = (int)ti0.size();
+                    if (cmb.ntiles[prec] > 0) {
+                        if (upload(h, &cmb.d_ti0[prec], ti0, true)) return 1;
+                        if (upload(h, &cmb.d_tj0[prec], tj0, true)) return 1;
+                        if (upload(h, &cmb.d_kbeg[prec], kb, true)) return 1;
+                        if (upload(h, &cmb.d_kend[prec], ke, true)) return 1;
                     }
                 }
-                cmb.ntiles[prec] = (int)ti0.size();
-                if (cmb.ntiles[prec] > 0) {
-                    if (upload(h, &cmb.d_ti0[prec], ti0, true)) return 1;
-                    if (upload(h, &cmb.d_tj0[prec], tj0, true)) return 1;
-                    if (upload(h, &cmb.d_kbeg[prec], kb, true)) return 1;
-                    if (upload(h, &cmb.d_kend[prec], ke, true)) return 1;
+                // row mode when the K rows of the bra shells fit in LDS
+                {
+                    const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
+                    cmb.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
+                    cmb.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
+                    const size_t need = hfx_row_lds_bytes(cb, ck, cmb.ncols);
+                    const int nr = (pair_class_la(cb) + pair_class_lb(cb) + lc + ld) / 2 + 1;
+                    cmb.rowmode = h->opts.kernel_mode == 0 && need > 0 && need <= h->lds_budget;
+                    cmb.boys_lds = nr <= 2 && need + BOYS_SIZE * sizeof(double) <= h->lds_budget;
+                    if (cmb.rowmode) {
+                        for (int prec = 0; prec < 2; prec++) {
+                            // ket chunk per block: whole rows when there are enough of them
+                            double tot = 0;
+                            int nrows = 0;
+                            for (int i = 0; i < nb; i++) {
+                                const int len = prec == 0 ? k64[i] : kmax[i] - k64[i];
+                                if (len > 0) {
+                                    tot += len;
+                                    nrows++;
+                                }
+                            }
+                            if (nrows == 0) continue;
+                            int chunk = 1 << 20;
+                            const double target_blocks = 4096;
+                            if (nrows < target_blocks) chunk = std::max(256, (int)std::ceil(tot / target_blocks / 64.0) * 64);
+                            std::vector<int> br, bk;
+                            for (int i = 0; i < nb; i++) {
+                                const int b0 = prec == 0 ? 0 : k64[i], b1 = prec == 0 ? k64[i] : kmax[i];
+                                for (int k0 = b0; k0 < b1; k0 += chunk) {
+                                    br.push_back(i);
+                                    bk.push_back(k0);
+                                }
+                            }
+                            cmb.chunk[prec] = chunk;
+                            cmb.nblk[prec] = (int)br.size();
+                            if (upload(h, &cmb.d_blk_row[prec], br, true)) return 1;
+                            if (upload(h, &cmb.d_blk_k0[prec], bk, true)) return 1;
+                        }
+                    }
                 }
+                if (cmb.ntiles[0] + cmb.ntiles[1] > 0) h->combos.push_back(cmb);
             }
-            h->combos.push_back(cmb);
         }
     if (h->opts.verbose) {
         double q64 = 0, q32 = 0;
@@ -423,6 +505,11 @@ int build_lists(hfx_t* h) {
         }
         fprintf(stderr, "hfx: %d shells, %zu pairs (%.0f significant), quartets fp64 %.4g fp32 %.4g\n",
                 (int)h->shells.size(), h->pairs.size(), h->npairs_sig, q64, q32);
+        for (auto& c : h->combos)
+            fprintf(stderr, "hfx:   class (%d%d|%d%d) ket np %d %s: quartets %.4g / %.4g, prim quartets %.4g / %.4g\n",
+                    pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck),
+                    c.npk, c.rowmode ? "rows" : "tiles", c.nq[0], c.nq[1],
+                    c.nprimq[0], c.nprimq[1]);
     }
     return 0;
 }
@@ -605,6 +692,7 @@ int hfx_default_opts(hfx_opts* o) {
     o->fp32_threshold = 0.0;
     o->device = 0;
     o->verbose = 0;
+    o->kernel_mode = 0;
     return 0;
 }
 
@@ -624,6 +712,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
         return fail(h, "hfx_create: invalid arguments (nbf=%d nat=%d)", nbf, nat);
     if (!(h->opts.schwarz_threshold >= 0) || !(h->opts.fp32_threshold >= 0))
         return fail(h, "hfx_create: thresholds must be >= 0");
+    if (h->opts.kernel_mode < 0 || h->opts.kernel_mode > 1) return fail(h, "hfx_create: kernel_mode must be 0 or 1");
     h->nat = nat;
     h->atno.assign(atno, atno + nat);
     h->xyz.assign(xyz, xyz + 3 * nat);
@@ -638,6 +727,8 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     HIPCHK(h, hipEventCreate(&h->ev1));
     const size_t nn = (size_t)nbf * nbf;
     if (upload(h, &h->d_scale, h->scale, false)) return 1;
+    for (int L = 0; L <= HFX_LMAX; L++)
+        if (upload(h, &h->d_bfL[L], h->bfL[L], false)) return 1;
     if (dmalloc(h, (void**)&h->d_D, nn * sizeof(double), false)) return 1;
     if (dmalloc(h, (void**)&h->d_Jb, nn * sizeof(double), false)) return 1;
     if (dmalloc(h, (void**)&h->d_Kb, nn * sizeof(double), false)) return 1;
@@ -668,20 +759,44 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     for (const auto& c : h->combos) {
         a.bra_list = h->d_cls_list[c.cb];
-        a.ket_list = h->d_cls_list[c.ck];
+        a.ket_list = c.d_ket;
         a.nbra = (int)h->cls_list[c.cb].size();
-        a.same = c.cb == c.ck;
         const FlopModel fm =
             flop_model(pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck));
         const double nint = (double)ncart(pair_class_la(c.cb)) * ncart(pair_class_lb(c.cb)) *
                             ncart(pair_class_la(c.ck)) * ncart(pair_class_lb(c.ck));
         for (int prec = 0; prec < 2; prec++) {
             if (c.ntiles[prec] == 0) continue;
-            a.tile_i0 = c.d_ti0[prec];
-            a.tile_j0 = c.d_tj0[prec];
-            a.kbeg = c.d_kbeg[prec];
-            a.kend = c.d_kend[prec];
-            HIPCHK(h, hfx_launch_jk(c.cb, c.ck, prec == 1, a, c.ntiles[prec], h->stream));
+            if (c.rowmode) {
+                if (c.nblk[prec] == 0) continue;
+                RowArgs r;
+                r.pairs = h->d_pairs;
+                r.prims = h->d_prims;
+                r.bra_list = a.bra_list;
+                r.ket_list = a.ket_list;
+                r.kbeg = c.d_kbeg[prec];
+                r.kend = c.d_kend[prec];
+                r.blk_row = c.d_blk_row[prec];
+                r.blk_k0 = c.d_blk_k0[prec];
+                r.chunk = c.chunk[prec];
+                r.ncols = c.ncols;
+                r.offd = c.offd;
+                const int lc = pair_class_la(c.ck), ld = pair_class_lb(c.ck);
+                r.bf_c = h->d_bfL[lc];
+                r.bf_d = h->d_bfL[ld];
+                r.boys_lds = c.boys_lds;
+                r.D = h->d_D;
+                r.J = h->d_Jb;
+                r.K = h->d_Kb;
+                r.n = n;
+                HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], h->stream));
+            } else {
+                a.tile_i0 = c.d_ti0[prec];
+                a.tile_j0 = c.d_tj0[prec];
+                a.kbeg = c.d_kbeg[prec];
+                a.kend = c.d_kend[prec];
+                HIPCHK(h, hfx_launch_jk(c.cb, c.ck, prec == 1, a, c.ntiles[prec], h->stream));
+            }
             const double fl = c.nprimq[prec] * fm.per_prim + c.nq[prec] * nint * fm.per_int;
             if (prec == 0) {
                 q64 += c.nq[0];
@@ -938,6 +1053,8 @@ void hfx_destroy(hfx_t* h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     free_geometry(h);
+    for (int L = 0; L <= HFX_LMAX; L++)
+        if (h->d_bfL[L]) (void)hipFree(h->d_bfL[L]);
     void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K};
     for (void* p : bufs)
         if (p) (void)hipFree(p);

@@ -10,7 +10,9 @@
 // Y (columns), X >= Y; both lists sorted by Schwarz bound, descending, so the
 // significant kets of a row form a prefix. Row i evaluates columns
 // [kbeg[i], kend[i]) (one precision's share of that prefix; for X == Y the
-// prefix is cut at the diagonal). Tiles: 8 rows x 32 columns, 256 threads.
+// prefix is cut at the diagonal). The ket list is one of the class's
+// sublists of equal primitive-pair count, so the primitive loops of a
+// wavefront have one trip count. Tiles: 4 rows x 64 columns, 256 threads.
 struct JKArgs {
     const ShellPair* pairs;
     const PrimPair* prims;
@@ -21,9 +23,37 @@ struct JKArgs {
     const int* kbeg;
     const int* kend;
     int nbra;
-    int same;   // X == Y
     const double* D;  // scaled, symmetrised density, n x n
     double* J;        // unsymmetrised accumulators, n x n
+    double* K;
+    int n;
+};
+
+// Arguments of one row-mode J/K launch: block b owns bra row blk_row[b] and
+// the ket columns [blk_k0[b], blk_k0[b] + chunk) of that row (intersected
+// with [kbeg, kend) of the row). The K contributions of the row land in the
+// rows of shells A and B, accumulated in LDS over all the block's kets and
+// flushed once: LDS columns are the functions of the L_C shells (then, when
+// L_D != L_C, those of the L_D shells from column offd on); bf_c / bf_d map
+// columns back to basis functions. J_ab is reduced over the block; only J_cd
+// goes to HBM per quartet.
+struct RowArgs {
+    const ShellPair* pairs;
+    const PrimPair* prims;
+    const int* bra_list;
+    const int* ket_list;
+    const int* kbeg;
+    const int* kend;
+    const int* blk_row;
+    const int* blk_k0;
+    int chunk;
+    int ncols;
+    int offd;
+    const int* bf_c;
+    const int* bf_d;
+    int boys_lds;   // one- and two-root classes: stage the Boys table in LDS too
+    const double* D;
+    double* J;
     double* K;
     int n;
 };
@@ -43,10 +73,13 @@ struct BlockArgs {
     double* out;
 };
 
-static constexpr int HFX_TILE_ROWS = 8;
-static constexpr int HFX_TILE_COLS = 32;
+static constexpr int HFX_TILE_ROWS = 4;   // one bra row per wavefront
+static constexpr int HFX_TILE_COLS = 64;
 
 hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, int ntiles, hipStream_t s);
+// LDS bytes of a row-mode launch for ncols columns, without the optional Boys table
+size_t hfx_row_lds_bytes(int cls_bra, int cls_ket, int ncols);
+hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, const RowArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
 // Q2[pair] = max over the diagonal integrals (ab|ab) of the pair (squared bound)
 hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,

@@ -16,7 +16,68 @@ struct RysTables {
     const double* cheb;  // RYS_CHEB
     const double* hu;    // RYS_HERM_U
     const double* hw;    // RYS_HERM_W
+    const double* boys;  // BOYS_TAB (global memory, or a copy in LDS)
 };
+
+HFX_HD HFX_INLINE double hfx_rsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ::rsqrt(x);
+#else
+    return 1.0 / std::sqrt(x);
+#endif
+}
+
+// One- and two-root rules in closed form from the Boys moments F_0..F_{2n-1}.
+// F_m(T) for T < BOYS_TMAX: Taylor series about the nearest grid point T0,
+//   F_m(T) = sum_{k < BOYS_K} F_{m+k}(T0) (T0 - T)^k / k!;
+// beyond: F_0 = (1/2) sqrt(pi/T) (erfc(sqrt T) < 1e-16 F_0 there) and the
+// upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable for T > m.
+// n = 1: u = F_1/F_0, w = F_0. n = 2: the monic orthogonal quadratic
+// u^2 + a u + b of the moment sequence, roots u_2 = (-a + sqrt(a^2 - 4b))/2,
+// u_1 = b / u_2, weights w_1 = (F_1 - u_2 F_0)/(u_1 - u_2), w_2 = F_0 - w_1.
+template <int NR>
+HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const double* boys) {
+    static_assert(NR == 1 || NR == 2, "closed form for one or two roots only");
+    constexpr int NM = 2 * NR;  // moments F_0 .. F_{2NR-1}
+    double F[NM];
+    if (T < BOYS_TMAX) {
+        const int i = (int)(T * BOYS_INV_DT + 0.5);
+        const double x = i * BOYS_DT - T;
+        double xk[BOYS_K];
+#pragma unroll
+        for (int k = 1; k < BOYS_K; k++) xk[k] = x * (1.0 / k);
+        const double* row = boys + i * BOYS_NCOL;
+#pragma unroll
+        for (int m = 0; m < NM; m++) {
+            double acc = row[m + BOYS_K - 1];
+#pragma unroll
+            for (int k = BOYS_K - 2; k >= 0; k--) acc = row[m + k] + acc * xk[k + 1];
+            F[m] = acc;
+        }
+    } else {
+        const double et = exp(-T);
+        const double it2 = 0.5 / T;
+        F[0] = 0.5 * sqrt(HFX_PI / T);
+#pragma unroll
+        for (int m = 0; m + 1 < NM; m++) F[m + 1] = ((2 * m + 1) * F[m] - et) * it2;
+    }
+    if (NR == 1) {
+        u[0] = F[1] / F[0];
+        w[0] = F[0];
+    } else {
+        const double m0 = F[0], m1 = F[1], m2 = F[NM > 2 ? 2 : 0], m3 = F[NM > 3 ? 3 : 0];
+        const double id = 1.0 / (m0 * m2 - m1 * m1);
+        const double a = (m1 * m2 - m0 * m3) * id;
+        const double b = (m1 * m3 - m2 * m2) * id;
+        const double u2 = 0.5 * (-a + sqrt(a * a - 4.0 * b));
+        const double u1 = b / u2;
+        const double w1 = (m1 - u2 * m0) / (u1 - u2);
+        u[0] = u1;
+        w[0] = w1;
+        u[NR - 1] = u2;
+        w[NR - 1] = m0 - w1;
+    }
+}
 
 // Offset (in doubles) of the n-root block of RYS_CHEB.
 HFX_HD constexpr int rys_offset(int n) {

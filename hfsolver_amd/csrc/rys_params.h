@@ -7,3 +7,10 @@
 #define RYS_H 1
 #define RYS_TMAX 60
 #define RYS_CHEB_SIZE 25200
+#define BOYS_DT 0.125
+#define BOYS_INV_DT 8
+#define BOYS_TMAX 36
+#define BOYS_K 9
+#define BOYS_NCOL 12
+#define BOYS_NPT 289
+#define BOYS_SIZE 3468

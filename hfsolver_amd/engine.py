@@ -21,7 +21,7 @@ class HfxError(RuntimeError):
 
 class HfxOpts(ctypes.Structure):
     _fields_ = [("schwarz_threshold", ctypes.c_double), ("fp32_threshold", ctypes.c_double),
-                ("device", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("device", ctypes.c_int), ("verbose", ctypes.c_int), ("kernel_mode", ctypes.c_int)]
 
 
 class HfxStats(ctypes.Structure):
@@ -78,7 +78,7 @@ def _f64(a):
 class Engine:
     """One geometry's direct J/K engine (owns device memory until close())."""
 
-    def __init__(self, basis, schwarz_threshold=1e-12, fp32_threshold=0.0, device=0, verbose=False):
+    def __init__(self, basis, schwarz_threshold=1e-12, fp32_threshold=0.0, device=0, verbose=False, kernel_mode=0):
         L = lib()
         self.basis = basis
         opts = HfxOpts()
@@ -87,6 +87,7 @@ class Engine:
         opts.fp32_threshold = fp32_threshold
         opts.device = device
         opts.verbose = 1 if verbose else 0
+        opts.kernel_mode = kernel_mode
         h = ctypes.c_void_p()
         b = basis
         self._keep = [b.nprim, b.istart, b.x, b.y, b.z, b.l, b.m, b.n, b.coef, b.alpha, b.atno, b.xyz]

@@ -41,6 +41,8 @@ typedef struct {
                                  0 = all FP64 (default 0) */
     int device;               /* HIP device ordinal (default 0) */
     int verbose;              /* print set-up statistics to stderr (default 0) */
+    int kernel_mode;          /* 0: automatic (K rows accumulated in LDS where they fit),
+                                 1: per-quartet atomics only (default 0) */
 } hfx_opts;
 
 typedef struct {

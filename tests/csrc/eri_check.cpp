@@ -15,7 +15,9 @@ namespace chk_tables {
 
 namespace {
 
-RysTables tabs() { return RysTables{chk_tables::RYS_CHEB, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W}; }
+RysTables tabs() {
+    return RysTables{chk_tables::RYS_CHEB, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W, chk_tables::BOYS_TAB};
+}
 
 void make_pair(int la, int lb, const double* A, const double* B, int na, const double* aa, const double* ca, int nb,
                const double* ab, const double* cb, ShellPair& sp, std::vector<PrimPair>& prims) {
@@ -23,6 +25,7 @@ void make_pair(int la, int lb, const double* A, const double* B, int na, const d
     sp.B = 1;
     sp.fa = sp.fb = 0;
     sp.cls = pair_class(la, lb);
+    sp.cpa = sp.cpb = 0;
     sp.ABx = A[0] - B[0];
     sp.ABy = A[1] - B[1];
     sp.ABz = A[2] - B[2];
@@ -34,6 +37,8 @@ void make_pair(int la, int lb, const double* A, const double* B, int na, const d
             const double a = aa[i], b = ab[j], p = a + b;
             PrimPair pp;
             pp.p = p;
+            pp.ip = 1.0 / p;
+            pp.pad = 0.0;
             pp.Px = (a * A[0] + b * B[0]) / p;
             pp.Py = (a * A[1] + b * B[1]) / p;
             pp.Pz = (a * A[2] + b * B[2]) / p;
@@ -104,6 +109,8 @@ int chk_eri_quartet(int la, int lb, int lc, int ld, const double* A, const doubl
 
 double chk_rys(int nr, double T, double* u, double* w) {
     switch (nr) {
+        case -1: rys_roots_small<1>(T, u, w, chk_tables::BOYS_TAB); break;
+        case -2: rys_roots_small<2>(T, u, w, chk_tables::BOYS_TAB); break;
         case 1: rys_roots<1>(T, u, w, tabs()); break;
         case 2: rys_roots<2>(T, u, w, tabs()); break;
         case 3: rys_roots<3>(T, u, w, tabs()); break;

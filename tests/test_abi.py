@@ -36,7 +36,7 @@ def test_default_opts():
     import ctypes
     o = E.HfxOpts()
     assert E.lib().hfx_default_opts(ctypes.byref(o)) == 0
-    assert o.schwarz_threshold == 1e-12 and o.fp32_threshold == 0.0 and o.device == 0
+    assert o.schwarz_threshold == 1e-12 and o.fp32_threshold == 0.0 and o.device == 0 and o.kernel_mode == 0
 
 
 def test_create_without_gpu_fails_loudly():

@@ -53,11 +53,12 @@ def test_eri_blocks_match_oracle(case):
     eng.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("name", ["h2o_testgeom", "h2o_config0", "ch4", "water2_seed2"])
-def test_fock_2e_matches_reference(golden, name):
+def test_fock_2e_matches_reference(golden, name, mode):
     g = golden(name)
-    say("fock_2e vs reference", name)
-    eng = Engine(Basis(g["atno"], g["xyz"]))
+    say("fock_2e vs reference", name, "kernel mode", mode)
+    eng = Engine(Basis(g["atno"], g["xyz"]), kernel_mode=mode)
     G, J, K, st = eng.fock_2e(g["P"], want_jk=True)
     assert np.abs(J - g["J"]).max() < 1e-11
     assert np.abs(K - g["K"]).max() < 1e-11
@@ -138,6 +139,11 @@ def test_sixteen_waters_invariants():
     assert np.abs(J1 - J1.T).max() < 1e-13 * scale
     # tr(P J(P)) >= 0 (Coulomb energy of a positive semidefinite density)
     assert np.sum(P1 * J1) > 0
+    # the two digestion kernels (LDS rows / per-quartet atomics) agree
+    eng1 = Engine(b, kernel_mode=1)
+    G1b, J1b, K1b, _ = eng1.fock_2e(P1, want_jk=True)
+    eng1.close()
+    assert np.abs(J1b - J1).max() < 1e-12 * scale and np.abs(K1b - K1).max() < 1e-12 * scale
     # sampled shell-quartet blocks against the oracle
     ob = O.Basis(atno, xyz)
     ns, f0, Ls, _ = eng.shells()
