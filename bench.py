@@ -29,6 +29,48 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_baseline(eng, basis, nq_total, target_s=12.0, seed=0):
+    """The oracle (THO restatement of the reference's cints.c, all host threads)
+    timed on a uniform random sample of this build's screened shell quartets;
+    builds/s extrapolated by the quartet count."""
+    from oracle import oracle as O
+    A, B, Q = eng.pair_bounds()
+    ns, f0, Ls, _ = eng.shells()
+    qmax = Q.max()
+    sig = np.nonzero(Q * qmax >= 1e-12)[0]
+    rng = np.random.default_rng(seed)
+    ob = O.Basis(basis.atno, basis.xyz)
+    nc = lambda L: (L + 1) * (L + 2) // 2
+
+    def sample(k):
+        parts, have = [], 0
+        while have < k:
+            i = rng.choice(sig, 4 * k)
+            j = rng.choice(sig, 4 * k)
+            ok = Q[i] * Q[j] >= 1e-12
+            parts.append(np.stack([A[i[ok]], B[i[ok]], A[j[ok]], B[j[ok]]], axis=1))
+            have += int(ok.sum())
+        q = np.concatenate(parts)[:k]
+        ncv = np.array([nc(L) for L in range(3)], dtype=np.int32)
+        return np.ascontiguousarray(f0[q], dtype=np.int32), np.ascontiguousarray(ncv[Ls[q]], dtype=np.int32)
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    k = 64
+    while True:
+        f, nf = sample(k)
+        t0 = time.perf_counter()
+        O.eri_quartets_sum(ob, f, nf, threads)
+        t = time.perf_counter() - t0
+        if t >= target_s or k >= 1 << 25:
+            break
+        k = int(k * min(16.0, max(2.0, 1.2 * target_s / max(t, 1e-3))))
+    per_build = t * nq_total / k
+    log("  cpu baseline: %d sampled quartets in %.2f s on %d threads -> %.1f s per build" % (k, t, threads, per_build))
+    return {"value": 1.0 / per_build, "unit": "Fock builds/s", "cores": threads, "kind": "port",
+            "sample": "%d uniformly sampled screened shell quartets of this build, every integral by the oracle's "
+                      "THO restatement of cints.c (%.1f s), extrapolated to %.4g quartets" % (k, t, nq_total)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,7 +82,19 @@ def main():
     ap.add_argument("--fp32-threshold", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (known bytes) to scale HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 3}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured mode).", file=sys.stderr)
@@ -103,6 +157,8 @@ def main():
     dF = float(np.abs(Gmx - G64).max())
     E64 = 0.5 * float(np.sum(P * (2 * H + G64)))
 
+    cpu = None if args.no_cpu_baseline else cpu_baseline(e64, b, st64[0].quartets_fp64)
+
     jk64 = np.mean([s.jk_seconds for s in st64])
     jkmx = np.mean([s.jk_seconds for s in stmx])
     f64 = st64[0].flops_fp64
@@ -136,7 +192,7 @@ def main():
                      "unit": "TFLOP/s", "frac": roof64 / PEAK_FP64, "traffic": None,
                      "kernel": "jk_kernel (all class launches of one FP64 build)",
                      "mixed_frac_of_weighted_peak": frac_mx},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     return 0

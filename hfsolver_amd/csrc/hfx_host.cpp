@@ -70,7 +70,7 @@ struct hfx_engine {
     std::vector<int> bf_atom;
     std::vector<int> bfL[HFX_LMAX + 1];   // functions of the shells of each L, in shell order
     int* d_bfL[HFX_LMAX + 1] = {nullptr};
-    size_t lds_budget = 80 * 1024;
+    size_t lds_budget = 120 * 1024;   // row-mode LDS per block (the heaviest classes run one block per CU)
     // pairs
     std::vector<ShellPair> pairs;
     std::vector<PrimPair> prims;
@@ -1030,6 +1030,17 @@ int hfx_shells(const hfx_t* h, int* nshell, int* first_bf, int* L, int* atom) {
         if (first_bf) first_bf[i] = h->shells[i].f0;
         if (L) L[i] = h->shells[i].L;
         if (atom) atom[i] = h->shells[i].atom;
+    }
+    return 0;
+}
+
+int hfx_pair_bounds(const hfx_t* h, int* npairs, int* A, int* B, double* Q) {
+    if (!h || !npairs) return 1;
+    *npairs = (int)h->pairs.size();
+    for (size_t i = 0; i < h->pairs.size(); i++) {
+        if (A) A[i] = h->pairs[i].A;
+        if (B) B[i] = h->pairs[i].B;
+        if (Q) Q[i] = h->pairs[i].Q;
     }
     return 0;
 }

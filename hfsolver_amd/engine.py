@@ -47,6 +47,7 @@ SIGNATURES = {
     "hfx_eri_shells": (_I, [_P, _I, _P, _P]),
     "hfx_shells": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
     "hfx_screening": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "hfx_pair_bounds": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
     "hfx_last_error": (ctypes.c_char_p, [_P]),
     "hfx_destroy": (None, [_P]),
 }
@@ -185,6 +186,16 @@ class Engine:
         at = np.zeros(ns.value, np.int32)
         self._check(lib().hfx_shells(self._h, ctypes.byref(ns), _ptr(f0), _ptr(Ls), _ptr(at)))
         return ns.value, f0, Ls, at
+
+    def pair_bounds(self):
+        """(A, B, Q) of every shell pair (Schwarz bounds from the GPU set-up)."""
+        npr = ctypes.c_int()
+        self._check(lib().hfx_pair_bounds(self._h, ctypes.byref(npr), None, None, None))
+        A = np.zeros(npr.value, np.int32)
+        B = np.zeros(npr.value, np.int32)
+        Q = np.zeros(npr.value)
+        self._check(lib().hfx_pair_bounds(self._h, ctypes.byref(npr), _ptr(A), _ptr(B), _ptr(Q)))
+        return A, B, Q
 
     def screening(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

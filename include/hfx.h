@@ -125,6 +125,11 @@ int hfx_eri_shells(hfx_t *h, int nq, const int *shells, double *out);
  * angular momentum and atom (arrays of length nshell, may be NULL). */
 int hfx_shells(const hfx_t *h, int *nshell, int *first_bf, int *L, int *atom);
 
+/* All shell pairs of the current geometry with their Schwarz bounds
+ * Q_AB = sqrt(max |(ab|ab)|) computed on the GPU: *npairs on return; A, B, Q
+ * (arrays of nshell*(nshell+1)/2, may be NULL) receive the shells and bounds. */
+int hfx_pair_bounds(const hfx_t *h, int *npairs, int *A, int *B, double *Q);
+
 /* Statistics of the last set-up: number of significant shell pairs and of
  * screened shell quartets in FP64 / FP32. */
 int hfx_screening(const hfx_t *h, double *npairs, double *nq_fp64, double *nq_fp32);

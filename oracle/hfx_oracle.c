@@ -535,3 +535,26 @@ void orc_grad_1e(int nbf, const int *nprim, const int *istart, const double *x, 
         }
     free(dVc);
 }
+
+/* ------------------------------------------------ CPU baseline sample --- */
+/* Evaluates every contracted integral of nq shell quartets (quartet k spans
+ * the basis functions [f[4k+x], f[4k+x] + nf[4k+x]) for x = 0..3) with the THO
+ * primitives, and returns their sum (so nothing is optimised away). Used to
+ * time the reference algorithm on a sample of a large build. */
+double orc_eri_quartets(int nbf, const int *nprim, const int *istart, const double *x, const double *y,
+                        const double *z, const int *l, const int *m, const int *n, const double *coef,
+                        const double *alpha, int nq, const int *f, const int *nf, int nthreads) {
+    orc_basis B = mkbasis(nbf, nprim, istart, x, y, z, l, m, n, coef, alpha);
+    double total = 0.0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (int q = 0; q < nq; q++)
+        for (int a = 0; a < nf[4 * q]; a++)
+            for (int b = 0; b < nf[4 * q + 1]; b++)
+                for (int c = 0; c < nf[4 * q + 2]; c++)
+                    for (int d = 0; d < nf[4 * q + 3]; d++)
+                        total += eri_bf(&B, f[4 * q] + a, f[4 * q + 1] + b, f[4 * q + 2] + c, f[4 * q + 3] + d);
+    return total;
+}

@@ -62,6 +62,8 @@ def lib():
         L.orc_grad_2e.restype = None
         L.orc_grad_1e.argtypes = [i] + [p] * 10 + [p, i, p, p, p, p, p, p, p]
         L.orc_grad_1e.restype = None
+        L.orc_eri_quartets.argtypes = [i] + [p] * 10 + [i, p, p, i]
+        L.orc_eri_quartets.restype = d
         _lib = L
     return _lib
 
@@ -231,6 +233,14 @@ def intindex(i, j, k, l):
 
 def eri(B, i, j, k, l):
     return lib().orc_eri_bf(*B.cargs(), i, j, k, l)
+
+
+def eri_quartets_sum(B, first, nfun, nthreads=0):
+    """Sum of all integrals of the given shell quartets (first/nfun: [nq, 4] arrays
+    of first function and function count); used to time the THO restatement."""
+    f = np.ascontiguousarray(first, dtype=np.int32)
+    nf = np.ascontiguousarray(nfun, dtype=np.int32)
+    return lib().orc_eri_quartets(*B.cargs(), len(f), _ptr(f), _ptr(nf), int(nthreads))
 
 
 def jk(B, P, nthreads=0):
