@@ -34,16 +34,24 @@ def run(cmd):
 
 def build_library(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    jobs = [
-        DEVICE_FLAGS + ["-c", os.path.join(CSRC, "hfx_kernels.hip"), "-o", os.path.join(BUILD, "hfx_kernels.o")],
-        HOST_FLAGS + ["-c", os.path.join(CSRC, "hfx_host.cpp"), "-o", os.path.join(BUILD, "hfx_host.o")],
-    ]
-    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+    # the kernel source builds in parts (see its header), in parallel
+    parts = {1: [], 2: ["-DHFX_REAL_T=double"], 3: ["-DHFX_REAL_T=float"], 4: ["-DHFX_REAL_T=double"],
+             5: ["-DHFX_REAL_T=float"], 6: []}
+    objs = []
+    jobs = []
+    for part, extra in parts.items():
+        o = os.path.join(BUILD, "hfx_kernels_%d.o" % part)
+        objs.append(o)
+        jobs.append(DEVICE_FLAGS + ["-DHFX_PART=%d" % part] + extra +
+                    ["-c", os.path.join(CSRC, "hfx_kernels.hip"), "-o", o])
+    objs.append(os.path.join(BUILD, "hfx_host.o"))
+    jobs.append(HOST_FLAGS + ["-c", os.path.join(CSRC, "hfx_host.cpp"), "-o", objs[-1]])
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
         list(ex.map(lambda j: run([HIPCC] + j), jobs))
     tmp = LIB + ".tmp"
-    run([HIPCC, "-shared", "-fopenmp", "--offload-arch=" + ARCH, "-o", tmp,
-         os.path.join(BUILD, "hfx_kernels.o"), os.path.join(BUILD, "hfx_host.o"),
-         "-Wl,-rpath,/opt/rocm/lib:/opt/rocm/llvm/lib"])
+    run([HIPCC, "-shared", "-fopenmp", "--offload-arch=" + ARCH, "-o", tmp] + objs +
+        ["-Wl,-rpath,/opt/rocm/lib:/opt/rocm/llvm/lib"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

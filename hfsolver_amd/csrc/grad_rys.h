@@ -1,0 +1,278 @@
+// Derivative electron-repulsion integrals contracted with the two-particle
+// density, by Rys quadrature (host and device; no reference counterpart:
+// hfsolver has no gradient code).
+//
+// For one shell quartet (AB|CD) and its slice (a-, c-components as in
+// eri_quartet), accumulates
+//   g_X += sum_{abcd} gam_abcd d(ab|cd)/dX,   X = A_x..A_z, B_x..B_z, C_x..C_z
+// with the centre-derivative identity on every primitive,
+//   d/dA_x x_A^i e^{-a x_A^2} = 2a x_A^{i+1} e^{..} - i x_A^{i-1} e^{..},
+// so the x factor of d/dA_x (ab|cd) is 2 a I_x(i+1, ...) - i I_x(i-1, ...) with
+// I_x the Rys 2-D integrals of eri_rys.h built one unit higher on the
+// differentiated centre. The D derivative follows from translational
+// invariance: d/dD = -(d/dA + d/dB + d/dC). The three centres are processed one
+// after the other for each root so that only one extended table is live.
+#pragma once
+
+#include "eri_rys.h"
+
+// Extents of the tables of the pass for centre CEN (0: A, 1: B, 2: C).
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, int CEN>
+struct GPass {
+    static constexpr int LAB = LA + LB + (CEN < 2 ? 1 : 0);  // highest power built on A
+    static constexpr int LCD = LC + LD + (CEN == 2 ? 1 : 0); // highest power built on C
+    static constexpr int LBx = LB + (CEN == 1 ? 1 : 0);      // kept powers of B
+    // kept powers of A: all 0..LA(+1), or the three around the selected one
+    static constexpr int XA = SPA ? (CEN == 0 ? 3 : 1) : (LA + 1 + (CEN == 0 ? 1 : 0));
+    static constexpr int XC = SPC ? (CEN == 2 ? 3 : 1) : (LC + 1 + (CEN == 2 ? 1 : 0));
+};
+
+// One direction of one root for the pass CEN: T[xa][ib][xc][id].
+// With SPA (SPC) the kept a (c) powers are ea-1, ea, ea+1 for the
+// differentiated centre (index 0, 1, 2; ea-1 < 0 reads 0 and is never used
+// with a non-zero weight) and ea alone otherwise.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, int CEN, typename R>
+HFX_HD HFX_INLINE void grad_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R g00, int ea, int ec,
+                                R (&T)[GPass<LA, LB, LC, LD, SPA, SPC, CEN>::XA][GPass<LA, LB, LC, LD, SPA, SPC, CEN>::LBx + 1]
+                                      [GPass<LA, LB, LC, LD, SPA, SPC, CEN>::XC][LD + 1]) {
+    using P = GPass<LA, LB, LC, LD, SPA, SPC, CEN>;
+    constexpr int NAB = P::LAB, NCD = P::LCD, LBx = P::LBx;
+    constexpr int LAk = LA + (CEN == 0 ? 1 : 0);  // highest kept a power
+    constexpr int LCk = LC + (CEN == 2 ? 1 : 0);
+    R G[NAB + 1][NCD + 1];
+    G[0][0] = g00;
+    if (NAB > 0) G[1][0] = c00 * g00;
+#pragma unroll
+    for (int n = 1; n < NAB; n++) G[n + 1][0] = c00 * G[n][0] + R(n) * b10 * G[n - 1][0];
+#pragma unroll
+    for (int m = 0; m < NCD; m++) {
+#pragma unroll
+        for (int n = 0; n <= NAB; n++) {
+            R v = d00 * G[n][m];
+            if (m > 0) v += R(m) * b01 * G[n][m - 1];
+            if (n > 0) v += R(n) * b00 * G[n - 1][m];
+            G[n][m + 1] = v;
+        }
+    }
+    R H[NAB + 1][LBx + 1][NCD + 1];
+#pragma unroll
+    for (int ia = 0; ia <= NAB; ia++)
+#pragma unroll
+        for (int m = 0; m <= NCD; m++) H[ia][0][m] = G[ia][m];
+#pragma unroll
+    for (int ib = 1; ib <= LBx; ib++)
+#pragma unroll
+        for (int ia = 0; ia <= NAB - ib; ia++)
+#pragma unroll
+            for (int m = 0; m <= NCD; m++) H[ia][ib][m] = H[ia + 1][ib - 1][m] + ab * H[ia][ib - 1][m];
+    R HA[P::XA][LBx + 1][NCD + 1];
+#pragma unroll
+    for (int xa = 0; xa < P::XA; xa++)
+#pragma unroll
+        for (int ib = 0; ib <= LBx; ib++)
+#pragma unroll
+            for (int m = 0; m <= NCD; m++) {
+                if (SPA) {
+                    const int off = (CEN == 0) ? xa - 1 : 0;
+                    R v = R(0);
+#pragma unroll
+                    for (int t = 0; t <= LAk; t++) v = (ea + off == t) ? H[t][ib][m] : v;
+                    HA[xa][ib][m] = v;
+                } else {
+                    HA[xa][ib][m] = H[xa][ib][m];
+                }
+            }
+    R Kt[P::XA][LBx + 1][NCD + 1][LD + 1];
+#pragma unroll
+    for (int xa = 0; xa < P::XA; xa++)
+#pragma unroll
+        for (int ib = 0; ib <= LBx; ib++) {
+#pragma unroll
+            for (int ic = 0; ic <= NCD; ic++) Kt[xa][ib][ic][0] = HA[xa][ib][ic];
+#pragma unroll
+            for (int id = 1; id <= LD; id++)
+#pragma unroll
+                for (int ic = 0; ic <= NCD - id; ic++)
+                    Kt[xa][ib][ic][id] = Kt[xa][ib][ic + 1][id - 1] + cd * Kt[xa][ib][ic][id - 1];
+        }
+#pragma unroll
+    for (int xa = 0; xa < P::XA; xa++)
+#pragma unroll
+        for (int ib = 0; ib <= LBx; ib++)
+#pragma unroll
+            for (int xc = 0; xc < P::XC; xc++)
+#pragma unroll
+                for (int id = 0; id <= LD; id++) {
+                    if (SPC) {
+                        const int off = (CEN == 2) ? xc - 1 : 0;
+                        R v = R(0);
+#pragma unroll
+                        for (int t = 0; t <= LCk; t++) v = (ec + off == t) ? Kt[xa][ib][t][id] : v;
+                        T[xa][ib][xc][id] = v;
+                    } else {
+                        T[xa][ib][xc][id] = Kt[xa][ib][xc][id];
+                    }
+                }
+}
+
+// Contraction of one pass with gam for one root: adds to g[3*CEN + {0,1,2}].
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, int CEN, typename R>
+HFX_HD HFX_INLINE void grad_pass(const double* c00, const double* d00, R b10, R b01, R b00, const R* abv,
+                                 const R* cdv, R g00z, const int* sel_a, const int* sel_c, R two_exp,
+                                 const R (&gam)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], R* g) {
+    using P = GPass<LA, LB, LC, LD, SPA, SPC, CEN>;
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    R Tx[P::XA][P::LBx + 1][P::XC][LD + 1];
+    R Ty[P::XA][P::LBx + 1][P::XC][LD + 1];
+    R Tz[P::XA][P::LBx + 1][P::XC][LD + 1];
+    grad_dir<LA, LB, LC, LD, SPA, SPC, CEN, R>(R(c00[0]), R(d00[0]), b10, b01, b00, abv[0], cdv[0], R(1), sel_a[0],
+                                               sel_c[0], Tx);
+    grad_dir<LA, LB, LC, LD, SPA, SPC, CEN, R>(R(c00[1]), R(d00[1]), b10, b01, b00, abv[1], cdv[1], R(1), sel_a[1],
+                                               sel_c[1], Ty);
+    grad_dir<LA, LB, LC, LD, SPA, SPC, CEN, R>(R(c00[2]), R(d00[2]), b10, b01, b00, abv[2], cdv[2], g00z, sel_a[2],
+                                               sel_c[2], Tz);
+    R gx = R(0), gy = R(0), gz = R(0);
+#pragma unroll
+    for (int a = 0; a < Q::NAt; a++) {
+        // exponents of a: table index of power p is p (unsliced) or 1 (+-1) around the selection
+        const int ex = SPA ? sel_a[0] : cart_exp(LA, a, 0), ey = SPA ? sel_a[1] : cart_exp(LA, a, 1),
+                  ez = SPA ? sel_a[2] : cart_exp(LA, a, 2);
+        const int ax = SPA ? (CEN == 0 ? 1 : 0) : cart_exp(LA, a, 0);
+        const int ay = SPA ? (CEN == 0 ? 1 : 0) : cart_exp(LA, a, 1);
+        const int az = SPA ? (CEN == 0 ? 1 : 0) : cart_exp(LA, a, 2);
+#pragma unroll
+        for (int b = 0; b < Q::NB; b++) {
+            const int bx = cart_exp(LB, b, 0), by = cart_exp(LB, b, 1), bz = cart_exp(LB, b, 2);
+#pragma unroll
+            for (int c = 0; c < Q::NCt; c++) {
+                const int fx = SPC ? sel_c[0] : cart_exp(LC, c, 0), fy = SPC ? sel_c[1] : cart_exp(LC, c, 1),
+                          fz = SPC ? sel_c[2] : cart_exp(LC, c, 2);
+                const int cx = SPC ? (CEN == 2 ? 1 : 0) : cart_exp(LC, c, 0);
+                const int cy = SPC ? (CEN == 2 ? 1 : 0) : cart_exp(LC, c, 1);
+                const int cz = SPC ? (CEN == 2 ? 1 : 0) : cart_exp(LC, c, 2);
+#pragma unroll
+                for (int d = 0; d < Q::ND; d++) {
+                    const int dx = cart_exp(LD, d, 0), dy = cart_exp(LD, d, 1), dz = cart_exp(LD, d, 2);
+                    const R gm = gam[((a * Q::NB + b) * Q::NCt + c) * Q::ND + d];
+                    const R x0 = Tx[ax][bx][cx][dx], y0 = Ty[ay][by][cy][dy], z0 = Tz[az][bz][cz][dz];
+                    R dxv, dyv, dzv;
+                    if (CEN == 0) {
+                        dxv = two_exp * Tx[ax + 1][bx][cx][dx] - (ex > 0 ? R(ex) * Tx[ax > 0 ? ax - 1 : 0][bx][cx][dx] : R(0));
+                        dyv = two_exp * Ty[ay + 1][by][cy][dy] - (ey > 0 ? R(ey) * Ty[ay > 0 ? ay - 1 : 0][by][cy][dy] : R(0));
+                        dzv = two_exp * Tz[az + 1][bz][cz][dz] - (ez > 0 ? R(ez) * Tz[az > 0 ? az - 1 : 0][bz][cz][dz] : R(0));
+                    } else if (CEN == 1) {
+                        dxv = two_exp * Tx[ax][bx + 1][cx][dx] - (bx > 0 ? R(bx) * Tx[ax][bx > 0 ? bx - 1 : 0][cx][dx] : R(0));
+                        dyv = two_exp * Ty[ay][by + 1][cy][dy] - (by > 0 ? R(by) * Ty[ay][by > 0 ? by - 1 : 0][cy][dy] : R(0));
+                        dzv = two_exp * Tz[az][bz + 1][cz][dz] - (bz > 0 ? R(bz) * Tz[az][bz > 0 ? bz - 1 : 0][cz][dz] : R(0));
+                    } else {
+                        dxv = two_exp * Tx[ax][bx][cx + 1][dx] - (fx > 0 ? R(fx) * Tx[ax][bx][cx > 0 ? cx - 1 : 0][dx] : R(0));
+                        dyv = two_exp * Ty[ay][by][cy + 1][dy] - (fy > 0 ? R(fy) * Ty[ay][by][cy > 0 ? cy - 1 : 0][dy] : R(0));
+                        dzv = two_exp * Tz[az][bz][cz + 1][dz] - (fz > 0 ? R(fz) * Tz[az][bz][cz > 0 ? cz - 1 : 0][dz] : R(0));
+                    }
+                    gx += gm * dxv * y0 * z0;
+                    gy += gm * x0 * dyv * z0;
+                    gz += gm * x0 * y0 * dzv;
+                }
+            }
+        }
+    }
+    g[3 * CEN + 0] += gx;
+    g[3 * CEN + 1] += gy;
+    g[3 * CEN + 2] += gz;
+}
+
+// g[0..8] += sum_{abcd in slice} gam_abcd d(ab|cd)/d(A, B, C) for the quartet.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void grad_quartet(const ShellPair& bra, const ShellPair& ket, const PrimPair* __restrict__ prims,
+                                    const PrimExp* __restrict__ pexp, int sa, int sc, const RysTables& tab,
+                                    const R (&gam)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], double (&g)[9]) {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
+    int sel_a[3] = {0, 0, 0}, sel_c[3] = {0, 0, 0};
+    if (SPA) {
+#pragma unroll
+        for (int i = 0; i < Q::NA; i++)
+            if (sa == i) {
+                sel_a[0] = cart_exp(LA, i, 0);
+                sel_a[1] = cart_exp(LA, i, 1);
+                sel_a[2] = cart_exp(LA, i, 2);
+            }
+    }
+    if (SPC) {
+#pragma unroll
+        for (int i = 0; i < Q::NC; i++)
+            if (sc == i) {
+                sel_c[0] = cart_exp(LC, i, 0);
+                sel_c[1] = cart_exp(LC, i, 1);
+                sel_c[2] = cart_exp(LC, i, 2);
+            }
+    }
+    const R abv[3] = {R(bra.ABx), R(bra.ABy), R(bra.ABz)};
+    const R cdv[3] = {R(ket.ABx), R(ket.ABy), R(ket.ABz)};
+    R gl[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) gl[k] = R(0);
+    for (int i1 = 0; i1 < bra.np; i1++) {
+        const PrimPair P = prims[bra.p0 + i1];
+        const PrimExp EB = pexp[bra.p0 + i1];
+        for (int k1 = 0; k1 < ket.np; k1++) {
+            const PrimPair Qp = prims[ket.p0 + k1];
+            const PrimExp EK = pexp[ket.p0 + k1];
+            const double p = P.p, q = Qp.p, pq = p + q;
+            const double ip = P.ip, iq = Qp.ip;
+            const double isq = hfx_rsqrt(pq);
+            const double ipq = isq * isq;
+            const double rho = p * q * ipq;
+            const double PQ[3] = {P.Px - Qp.Px, P.Py - Qp.Py, P.Pz - Qp.Pz};
+            const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double pref = HFX_TWO_PI_52 * P.K * Qp.K * ip * iq * isq;
+            double u[NR], w[NR];
+            if constexpr (NR <= 2)
+                rys_roots_small<NR>(T, u, w, tab.boys);
+            else
+                rys_roots<NR>(T, u, w, tab);
+            const double PA[3] = {P.PAx, P.PAy, P.PAz}, QC[3] = {Qp.PAx, Qp.PAy, Qp.PAz};
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const double ur = u[r];
+                const double t1 = ur * rho;
+                const R b00 = R(0.5 * ur * ipq);
+                const R b10 = R(0.5 * ip * (1.0 - t1 * ip));
+                const R b01 = R(0.5 * iq * (1.0 - t1 * iq));
+                const double fp = t1 * ip, fq = t1 * iq;
+                const double c00[3] = {PA[0] - fp * PQ[0], PA[1] - fp * PQ[1], PA[2] - fp * PQ[2]};
+                const double d00[3] = {QC[0] + fq * PQ[0], QC[1] + fq * PQ[1], QC[2] + fq * PQ[2]};
+                const R wz = R(w[r] * pref);
+                grad_pass<LA, LB, LC, LD, SPA, SPC, 0, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
+                                                          R(2.0 * EB.aa), gam, gl);
+                grad_pass<LA, LB, LC, LD, SPA, SPC, 1, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
+                                                          R(2.0 * EB.ab), gam, gl);
+                grad_pass<LA, LB, LC, LD, SPA, SPC, 2, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
+                                                          R(2.0 * EK.aa), gam, gl);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] += double(gl[k]);
+}
+
+// gam_abcd = s (4 D_ab D_cd - D_ac D_bd - D_ad D_bc) for the thread's slice;
+// fa / fc are the first functions of the slice (fa + sa, fc + sc when slicing).
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void grad_gamma(int fa, int fb, int fc, int fd, const double* __restrict__ D, int n, double s,
+                                  R (&gam)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT]) {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+#pragma unroll
+    for (int a = 0; a < Q::NAt; a++)
+#pragma unroll
+        for (int b = 0; b < Q::NB; b++)
+#pragma unroll
+            for (int c = 0; c < Q::NCt; c++)
+#pragma unroll
+                for (int d = 0; d < Q::ND; d++) {
+                    const double dab = D[(size_t)(fa + a) * n + fb + b], dcd = D[(size_t)(fc + c) * n + fd + d];
+                    const double dac = D[(size_t)(fa + a) * n + fc + c], dbd = D[(size_t)(fb + b) * n + fd + d];
+                    const double dad = D[(size_t)(fa + a) * n + fd + d], dbc = D[(size_t)(fb + b) * n + fc + c];
+                    gam[((a * Q::NB + b) * Q::NCt + c) * Q::ND + d] = R(s * (4.0 * dab * dcd - dac * dbd - dad * dbc));
+                }
+}

@@ -112,6 +112,8 @@ struct hfx_engine {
     double* d_G = nullptr;
     double* d_J = nullptr;
     double* d_K = nullptr;
+    int* d_shell_atom = nullptr;
+    double* d_grad = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void*> geom_allocs;   // freed on geometry change
 };
@@ -463,7 +465,7 @@ int build_lists(hfx_t* h) {
                     const int nr = (pair_class_la(cb) + pair_class_lb(cb) + lc + ld) / 2 + 1;
                     cmb.rowmode = h->opts.kernel_mode == 0 && need > 0 && need <= h->lds_budget;
                     cmb.boys_lds = nr <= 2 && need + BOYS_SIZE * sizeof(double) <= h->lds_budget;
-                    if (cmb.rowmode) {
+                    {   // row blocks: used by row-mode J/K and by the gradient kernel
                         for (int prec = 0; prec < 2; prec++) {
                             // ket chunk per block: whole rows when there are enough of them
                             double tot = 0;
@@ -729,6 +731,12 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (upload(h, &h->d_scale, h->scale, false)) return 1;
     for (int L = 0; L <= HFX_LMAX; L++)
         if (upload(h, &h->d_bfL[L], h->bfL[L], false)) return 1;
+    {
+        std::vector<int> sat(h->shells.size());
+        for (size_t i = 0; i < h->shells.size(); i++) sat[i] = h->shells[i].atom;
+        if (upload(h, &h->d_shell_atom, sat, false)) return 1;
+        if (dmalloc(h, (void**)&h->d_grad, 3 * (size_t)nat * sizeof(double), false)) return 1;
+    }
     if (dmalloc(h, (void**)&h->d_D, nn * sizeof(double), false)) return 1;
     if (dmalloc(h, (void**)&h->d_Jb, nn * sizeof(double), false)) return 1;
     if (dmalloc(h, (void**)&h->d_Kb, nn * sizeof(double), false)) return 1;
@@ -915,9 +923,44 @@ int hfx_grad_1e(hfx_t* h, const double* P, const double* W, double* g) {
 }
 
 int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
-    (void)P;
-    (void)g;
-    return h ? fail(h, "hfx_grad_2e: not available in this build") : 1;
+    if (!h || !P || !g) return h ? fail(h, "hfx_grad_2e: null argument") : 1;
+    HIPCHK(h, hipSetDevice(h->device));
+    const int n = h->nbf;
+    const size_t nn = (size_t)n * n;
+    if (!h->d_P) {
+        if (dmalloc(h, (void**)&h->d_P, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_G, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_J, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_K, nn * sizeof(double), false)) return 1;
+    }
+    HIPCHK(h, hipMemcpy(h->d_P, P, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hfx_launch_prep(h->d_P, h->d_scale, h->d_D, h->d_Jb, h->d_Kb, n, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_grad, 0, 3 * (size_t)h->nat * sizeof(double), h->stream));
+    GradArgs a;
+    a.pairs = h->d_pairs;
+    a.prims = h->d_prims;
+    a.pexp = h->d_pexp;
+    a.shell_atom = h->d_shell_atom;
+    a.nat = h->nat;
+    a.D = h->d_D;
+    a.grad = h->d_grad;
+    a.n = n;
+    for (const auto& c : h->combos) {
+        a.bra_list = h->d_cls_list[c.cb];
+        a.ket_list = c.d_ket;
+        for (int prec = 0; prec < 2; prec++) {
+            if (c.nblk[prec] == 0) continue;
+            a.kbeg = c.d_kbeg[prec];
+            a.kend = c.d_kend[prec];
+            a.blk_row = c.d_blk_row[prec];
+            a.blk_k0 = c.d_blk_k0[prec];
+            a.chunk = c.chunk[prec];
+            HIPCHK(h, hfx_launch_grad(c.cb, c.ck, prec == 1, a, c.nblk[prec], h->stream));
+        }
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(g, h->d_grad, 3 * (size_t)h->nat * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
 }
 
 int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
@@ -1066,7 +1109,7 @@ void hfx_destroy(hfx_t* h) {
     free_geometry(h);
     for (int L = 0; L <= HFX_LMAX; L++)
         if (h->d_bfL[L]) (void)hipFree(h->d_bfL[L]);
-    void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K};
+    void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K, h->d_shell_atom, h->d_grad};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);

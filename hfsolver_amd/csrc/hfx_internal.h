@@ -58,6 +58,27 @@ struct RowArgs {
     int n;
 };
 
+// Arguments of one gradient launch (same row blocks as RowArgs): every quartet's
+// sum_abcd gam_abcd d(ab|cd)/dR goes to the atoms of its four shells, summed in
+// LDS per block and flushed into grad[3 * nat].
+struct GradArgs {
+    const ShellPair* pairs;
+    const PrimPair* prims;
+    const PrimExp* pexp;
+    const int* bra_list;
+    const int* ket_list;
+    const int* kbeg;
+    const int* kend;
+    const int* blk_row;
+    const int* blk_k0;
+    int chunk;
+    const int* shell_atom;
+    int nat;
+    const double* D;
+    double* grad;
+    int n;
+};
+
 // Arguments of the explicit-quartet kernel: quartet q uses pairs bra[q], ket[q]
 // (already oriented l_A >= l_B, l_C >= l_D, class(bra) >= class(ket)) and
 // writes its block at out + off[q] in the (a, b, c, d) order of the oriented
@@ -80,6 +101,7 @@ hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, i
 // LDS bytes of a row-mode launch for ncols columns, without the optional Boys table
 size_t hfx_row_lds_bytes(int cls_bra, int cls_ket, int ncols);
 hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, const RowArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_grad(int cls_bra, int cls_ket, bool fp32, const GradArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
 // Q2[pair] = max over the diagonal integrals (ab|ab) of the pair (squared bound)
 hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,

@@ -10,14 +10,23 @@
 //                G = J - K/2.
 #include <hip/hip_runtime.h>
 
-#include "eri_rys.h"
+#include "grad_rys.h"
 #include "hfx_internal.h"
 
+// The file is compiled once per part (-DHFX_PART=1..6) so the parts build in
+// parallel: 1 tile-mode J/K, 2/3 row-mode J/K in FP64/FP32, 4/5 gradient in
+// FP64/FP32, 6 Schwarz, explicit quartets, prep/finish and the dispatchers.
+#ifndef HFX_PART
+#error "compile with -DHFX_PART=1..6"
+#endif
+
+namespace {
 #define RYS_TABLE_QUALIFIER __device__ const
 namespace hfx_dev {
 #include "rys_tables.inc"
 }
 #undef RYS_TABLE_QUALIFIER
+}  // namespace
 
 namespace {
 
@@ -31,6 +40,7 @@ __device__ HFX_INLINE double wave_sum(double v) {
     return v;
 }
 
+#if HFX_PART == 1
 // One tile = HFX_TILE_ROWS bra rows x HFX_TILE_COLS ket columns; each
 // wavefront owns one bra row (one bra shell pair), so J_ab is reduced across
 // the wavefront before a single atomic per element.
@@ -103,7 +113,9 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
         for (int d = 0; d < Q::ND; d++) unsafeAtomicAdd(&a.K[(size_t)(fb + b) * n + fd + d], s * double(acc.Kbd[b][d]));
     }
 }
+#endif
 
+#if HFX_PART == 2 || HFX_PART == 3
 // Row mode: one block = one bra shell pair x a chunk of its kets (see RowArgs).
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
@@ -201,7 +213,68 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
         if (v != 0.0) unsafeAtomicAdd(&a.J[(size_t)(fa + k / NB) * n + fb + k % NB], v);
     }
 }
+#endif
 
+#if HFX_PART == 4 || HFX_PART == 5
+// Two-electron gradient: one block = one bra shell pair x a chunk of its kets.
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
+    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    constexpr int NRG = (LA + LB + LC + LD + 1) / 2 + 1;
+    extern __shared__ double smem[];
+    double* gl = smem;  // 3 * nat
+    const double* boys = hfx_dev::BOYS_TAB;
+    if constexpr (NRG <= 2) {
+        double* bl = smem + 3 * a.nat;
+        for (int k = threadIdx.x; k < BOYS_SIZE; k += blockDim.x) bl[k] = hfx_dev::BOYS_TAB[k];
+        boys = bl;
+    }
+    for (int k = threadIdx.x; k < 3 * a.nat; k += blockDim.x) gl[k] = 0.0;
+    __syncthreads();
+    const int row = a.blk_row[blockIdx.x];
+    const int split = blockIdx.y;
+    const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
+    const int sc = SPC ? split % Q::NC : 0;
+    const int bp = a.bra_list[row];
+    const ShellPair bra = a.pairs[bp];
+    const int k0 = a.blk_k0[blockIdx.x];
+    const int j0 = max(k0, a.kbeg[row]);
+    const int j1 = min(k0 + a.chunk, a.kend[row]);
+    const double sbra = bra.A == bra.B ? 0.5 : 1.0;
+    double gab[6] = {0, 0, 0, 0, 0, 0};
+    for (int col = j0 + (int)threadIdx.x; col < j1; col += (int)blockDim.x) {
+        const int kp = a.ket_list[col];
+        const ShellPair ket = a.pairs[kp];
+        double s = sbra;
+        if (ket.A == ket.B) s *= 0.5;
+        if (bp == kp) s *= 0.5;
+        R gam[Q::NOUT];
+        grad_gamma<LA, LB, LC, LD, SPA, SPC, R>(bra.fa + sa, bra.fb, ket.fa + sc, ket.fb, a.D, a.n, s, gam);
+        double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        grad_quartet<LA, LB, LC, LD, SPA, SPC, R>(bra, ket, a.prims, a.pexp, sa, sc, dev_tables(boys), gam, g);
+#pragma unroll
+        for (int k = 0; k < 6; k++) gab[k] += g[k];
+        const int atc = a.shell_atom[ket.A], atd = a.shell_atom[ket.B];
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            atomicAdd(&gl[3 * atc + x], g[6 + x]);
+            atomicAdd(&gl[3 * atd + x], -(g[x] + g[3 + x] + g[6 + x]));
+        }
+    }
+    const int ata = a.shell_atom[bra.A], atb = a.shell_atom[bra.B];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const double v = wave_sum(gab[k]);
+        if ((threadIdx.x % 64) == 0 && v != 0.0) atomicAdd(&gl[3 * (k < 3 ? ata : atb) + k % 3], v);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * a.nat; k += blockDim.x)
+        if (gl[k] != 0.0) unsafeAtomicAdd(&a.grad[k], gl[k]);
+}
+#endif
+
+#if HFX_PART == 6
 // q2bits[pair] = bits of max_{a,b} |(ab|ab)| (non-negative doubles order as integers)
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void schwarz_kernel(const ShellPair* pairs, const PrimPair* prims, const int* list,
@@ -285,90 +358,142 @@ __global__ void finish_kernel(const double* __restrict__ Jb, const double* __res
         if (K) K[t] = kv;
     }
 }
-
-template <int LA, int LB, int LC, int LD, typename R>
-hipError_t launch_jk_t(const JKArgs& a, int ntiles, hipStream_t s) {
-    dim3 grid((unsigned)ntiles, (unsigned)nsplit(LA, LB, LC, LD));
-    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), grid, dim3(HFX_TILE_ROWS * HFX_TILE_COLS), 0, s, a);
-    return hipGetLastError();
-}
+#endif
 
 template <int LA, int LB, int LC, int LD>
 constexpr int row_nab() {
     return (split_a(LA, LB, LC, LD) ? 1 : ncart(LA)) + ncart(LB);
 }
 
-template <int LA, int LB, int LC, int LD, typename R>
-hipError_t launch_rows_t(const RowArgs& a, int nblocks, hipStream_t s) {
-    constexpr int NAt = split_a(LA, LB, LC, LD) ? 1 : ncart(LA);
-    const size_t lds = ((size_t)row_nab<LA, LB, LC, LD>() * a.ncols + NAt * ncart(LB) + (a.boys_lds ? BOYS_SIZE : 0)) *
-                       sizeof(double);
+// table of a function template F<X, Y> over class combinations X >= Y
+#define HFX_TROW(F, X)                                                                                    \
+    {F<X, 0>, X >= 1 ? F<X, (X >= 1 ? 1 : 0)> : nullptr, X >= 2 ? F<X, (X >= 2 ? 2 : 0)> : nullptr,        \
+     X >= 3 ? F<X, (X >= 3 ? 3 : 0)> : nullptr, X >= 4 ? F<X, (X >= 4 ? 4 : 0)> : nullptr,                 \
+     X >= 5 ? F<X, (X >= 5 ? 5 : 0)> : nullptr}
+#define HFX_TABLE(F) {HFX_TROW(F, 0), HFX_TROW(F, 1), HFX_TROW(F, 2), HFX_TROW(F, 3), HFX_TROW(F, 4), HFX_TROW(F, 5)}
+#define HFX_CLS(X) pair_class_la(X), pair_class_lb(X)
+
+#if HFX_PART == 1
+template <int X, int Y>
+hipError_t jk_xy(bool fp32, const JKArgs& a, int ntiles, hipStream_t s) {
+    dim3 grid((unsigned)ntiles, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    if (fp32)
+        hipLaunchKernelGGL((jk_kernel<HFX_CLS(X), HFX_CLS(Y), float>), grid, dim3(HFX_TILE_ROWS * HFX_TILE_COLS), 0, s, a);
+    else
+        hipLaunchKernelGGL((jk_kernel<HFX_CLS(X), HFX_CLS(Y), double>), grid, dim3(HFX_TILE_ROWS * HFX_TILE_COLS), 0, s,
+                           a);
+    return hipGetLastError();
+}
+typedef hipError_t (*jk_fn)(bool, const JKArgs&, int, hipStream_t);
+const jk_fn JK_TABLE[6][6] = HFX_TABLE(jk_xy);
+#endif
+
+#if HFX_PART == 2 || HFX_PART == 3
+typedef HFX_REAL_T RowReal;
+template <int X, int Y>
+hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
+    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
+    const size_t lds = ((size_t)row_nab<HFX_CLS(X), HFX_CLS(Y)>() * a.ncols + NAt * ncart(pair_class_lb(X)) +
+                        (a.boys_lds ? BOYS_SIZE : 0)) * sizeof(double);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)jk_row_kernel<LA, LB, LC, LD, R>,
+        hipError_t e = hipFuncSetAttribute((const void*)jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 32 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    dim3 grid((unsigned)nblocks, (unsigned)nsplit(LA, LB, LC, LD));
-    hipLaunchKernelGGL((jk_row_kernel<LA, LB, LC, LD, R>), grid, dim3(256), lds, s, a);
+    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    hipLaunchKernelGGL((jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }
+typedef hipError_t (*rows_fn)(const RowArgs&, int, hipStream_t);
+const rows_fn ROWS_TABLE[6][6] = HFX_TABLE(rows_xy);
+#endif
 
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_block_t(const BlockArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((block_kernel<LA, LB, LC, LD>), dim3((unsigned)a.nq), dim3(64), 0, s, a);
-    return hipGetLastError();
-}
-
+#if HFX_PART == 4 || HFX_PART == 5
+typedef HFX_REAL_T GradReal;
 template <int X, int Y>
-struct Combo {
-    static constexpr int LA = pair_class_la(X), LB = pair_class_lb(X), LC = pair_class_la(Y), LD = pair_class_lb(Y);
-    static hipError_t jk(bool fp32, const JKArgs& a, int nt, hipStream_t s) {
-        return fp32 ? launch_jk_t<LA, LB, LC, LD, float>(a, nt, s) : launch_jk_t<LA, LB, LC, LD, double>(a, nt, s);
+hipError_t grad_xy(const GradArgs& a, int nblocks, hipStream_t s) {
+    constexpr int NRG = (pair_class_la(X) + pair_class_lb(X) + pair_class_la(Y) + pair_class_lb(Y) + 1) / 2 + 1;
+    const size_t lds = (3 * (size_t)a.nat + (NRG <= 2 ? BOYS_SIZE : 0)) * sizeof(double);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)grad_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 32 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
     }
-    static hipError_t blk(const BlockArgs& a, hipStream_t s) { return launch_block_t<LA, LB, LC, LD>(a, s); }
-    static hipError_t rows(bool fp32, const RowArgs& a, int nb, hipStream_t s) {
-        return fp32 ? launch_rows_t<LA, LB, LC, LD, float>(a, nb, s) : launch_rows_t<LA, LB, LC, LD, double>(a, nb, s);
-    }
-    static size_t row_lds(int ncols) {  // without the optional Boys table
-        constexpr int NAt = split_a(LA, LB, LC, LD) ? 1 : ncart(LA);
-        return ((size_t)row_nab<LA, LB, LC, LD>() * ncols + NAt * ncart(LB)) * sizeof(double);
-    }
-};
+    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    hipLaunchKernelGGL((grad_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>), grid, dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+typedef hipError_t (*grad_fn)(const GradArgs&, int, hipStream_t);
+const grad_fn GRAD_TABLE[6][6] = HFX_TABLE(grad_xy);
+#endif
 
-typedef hipError_t (*jk_fn)(bool, const JKArgs&, int, hipStream_t);
+#if HFX_PART == 6
+template <int X, int Y>
+hipError_t block_xy(const BlockArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((block_kernel<HFX_CLS(X), HFX_CLS(Y)>), dim3((unsigned)a.nq), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+template <int X, int Y>
+size_t row_lds_xy(int ncols) {  // without the optional Boys table
+    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
+    return ((size_t)row_nab<HFX_CLS(X), HFX_CLS(Y)>() * ncols + NAt * ncart(pair_class_lb(X))) * sizeof(double);
+}
 typedef hipError_t (*blk_fn)(const BlockArgs&, hipStream_t);
-typedef hipError_t (*rows_fn)(bool, const RowArgs&, int, hipStream_t);
 typedef size_t (*lds_fn)(int);
-
-#define HFX_ROW(X)                                                                                         \
-    {Combo<X, 0>::jk, X >= 1 ? Combo<X, (X >= 1 ? 1 : 0)>::jk : nullptr,                                   \
-     X >= 2 ? Combo<X, (X >= 2 ? 2 : 0)>::jk : nullptr, X >= 3 ? Combo<X, (X >= 3 ? 3 : 0)>::jk : nullptr,  \
-     X >= 4 ? Combo<X, (X >= 4 ? 4 : 0)>::jk : nullptr, X >= 5 ? Combo<X, (X >= 5 ? 5 : 0)>::jk : nullptr}
-#define HFX_BROW(X)                                                                                          \
-    {Combo<X, 0>::blk, X >= 1 ? Combo<X, (X >= 1 ? 1 : 0)>::blk : nullptr,                                   \
-     X >= 2 ? Combo<X, (X >= 2 ? 2 : 0)>::blk : nullptr, X >= 3 ? Combo<X, (X >= 3 ? 3 : 0)>::blk : nullptr,  \
-     X >= 4 ? Combo<X, (X >= 4 ? 4 : 0)>::blk : nullptr, X >= 5 ? Combo<X, (X >= 5 ? 5 : 0)>::blk : nullptr}
-
-const jk_fn JK_TABLE[6][6] = {HFX_ROW(0), HFX_ROW(1), HFX_ROW(2), HFX_ROW(3), HFX_ROW(4), HFX_ROW(5)};
-
-#define HFX_GROW(X, F)                                                                                       {Combo<X, 0>::F, X >= 1 ? Combo<X, (X >= 1 ? 1 : 0)>::F : nullptr,                                        X >= 2 ? Combo<X, (X >= 2 ? 2 : 0)>::F : nullptr, X >= 3 ? Combo<X, (X >= 3 ? 3 : 0)>::F : nullptr,       X >= 4 ? Combo<X, (X >= 4 ? 4 : 0)>::F : nullptr, X >= 5 ? Combo<X, (X >= 5 ? 5 : 0)>::F : nullptr}
-const rows_fn ROWS_TABLE[6][6] = {HFX_GROW(0, rows), HFX_GROW(1, rows), HFX_GROW(2, rows),
-                                  HFX_GROW(3, rows), HFX_GROW(4, rows), HFX_GROW(5, rows)};
-const lds_fn LDS_TABLE[6][6] = {HFX_GROW(0, row_lds), HFX_GROW(1, row_lds), HFX_GROW(2, row_lds),
-                                HFX_GROW(3, row_lds), HFX_GROW(4, row_lds), HFX_GROW(5, row_lds)};
-const blk_fn BLK_TABLE[6][6] = {HFX_BROW(0), HFX_BROW(1), HFX_BROW(2), HFX_BROW(3), HFX_BROW(4), HFX_BROW(5)};
+const blk_fn BLK_TABLE[6][6] = HFX_TABLE(block_xy);
+const lds_fn LDS_TABLE[6][6] = HFX_TABLE(row_lds_xy);
+#endif
 
 }  // namespace
 
+#define HFX_CHECK_COMBO(cb, ck) \
+    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return hipErrorInvalidValue
+
+#if HFX_PART == 1
+hipError_t hfx_launch_jk(int cb, int ck, bool fp32, const JKArgs& a, int ntiles, hipStream_t s) {
+    HFX_CHECK_COMBO(cb, ck);
+    if (ntiles <= 0) return hipErrorInvalidValue;
+    return JK_TABLE[cb][ck](fp32, a, ntiles, s);
+}
+#endif
+
+#if HFX_PART == 2
+hipError_t hfx_launch_jk_rows_f64(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s) {
+    HFX_CHECK_COMBO(cb, ck);
+    return ROWS_TABLE[cb][ck](a, nblocks, s);
+}
+#endif
+#if HFX_PART == 3
+hipError_t hfx_launch_jk_rows_f32(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s) {
+    HFX_CHECK_COMBO(cb, ck);
+    return ROWS_TABLE[cb][ck](a, nblocks, s);
+}
+#endif
+#if HFX_PART == 4
+hipError_t hfx_launch_grad_f64(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s) {
+    HFX_CHECK_COMBO(cb, ck);
+    return GRAD_TABLE[cb][ck](a, nblocks, s);
+}
+#endif
+#if HFX_PART == 5
+hipError_t hfx_launch_grad_f32(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s) {
+    HFX_CHECK_COMBO(cb, ck);
+    return GRAD_TABLE[cb][ck](a, nblocks, s);
+}
+#endif
+
+#if HFX_PART == 6
+hipError_t hfx_launch_jk_rows_f64(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_jk_rows_f32(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_grad_f64(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_grad_f32(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s);
+
 int hfx_nsplit(int cb, int ck) {
     return nsplit(pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck));
-}
-
-hipError_t hfx_launch_jk(int cb, int ck, bool fp32, const JKArgs& a, int ntiles, hipStream_t s) {
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || ntiles <= 0) return hipErrorInvalidValue;
-    return JK_TABLE[cb][ck](fp32, a, ntiles, s);
 }
 
 size_t hfx_row_lds_bytes(int cb, int ck, int ncols) {
@@ -377,15 +502,20 @@ size_t hfx_row_lds_bytes(int cb, int ck, int ncols) {
 }
 
 hipError_t hfx_launch_jk_rows(int cb, int ck, bool fp32, const RowArgs& a, int nblocks, hipStream_t s) {
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || nblocks <= 0) return hipErrorInvalidValue;
-    return ROWS_TABLE[cb][ck](fp32, a, nblocks, s);
+    if (nblocks <= 0) return hipErrorInvalidValue;
+    return fp32 ? hfx_launch_jk_rows_f32(cb, ck, a, nblocks, s) : hfx_launch_jk_rows_f64(cb, ck, a, nblocks, s);
+}
+
+hipError_t hfx_launch_grad(int cb, int ck, bool fp32, const GradArgs& a, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipErrorInvalidValue;
+    return fp32 ? hfx_launch_grad_f32(cb, ck, a, nblocks, s) : hfx_launch_grad_f64(cb, ck, a, nblocks, s);
 }
 
 hipError_t hfx_launch_block(int cb, int ck, const BlockArgs& a, hipStream_t s) {
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || a.nq <= 0) return hipErrorInvalidValue;
+    HFX_CHECK_COMBO(cb, ck);
+    if (a.nq <= 0) return hipErrorInvalidValue;
     return BLK_TABLE[cb][ck](a, s);
 }
-
 hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,
                               unsigned long long* q2bits, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
@@ -438,3 +568,5 @@ hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* s
     hipLaunchKernelGGL(finish_kernel, dim3(nb), dim3(256), 0, s, Jb, Kb, scale, G, J, K, n);
     return hipGetLastError();
 }
+
+#endif  // HFX_PART == 6

@@ -81,7 +81,8 @@ def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=F
         if callback is not None:
             callback(it, E0, P, G)
         Fd = F
-        if diis:
+        if diis and np.any(P):
+            # the P = 0 start has a zero commutator and must not enter the subspace
             err = F @ P @ S - S @ P @ F
             Fs.append(F)
             errs.append(err)

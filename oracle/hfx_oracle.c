@@ -558,3 +558,11 @@ double orc_eri_quartets(int nbf, const int *nprim, const int *istart, const doub
                         total += eri_bf(&B, f[4 * q] + a, f[4 * q + 1] + b, f[4 * q + 2] + c, f[4 * q + 3] + d);
     return total;
 }
+
+/* d(ij|kl)/dR_c along axis ax for the four centres c = 0..3 (functions i, j, k, l). */
+void orc_deri_bf(int nbf, const int *nprim, const int *istart, const double *x, const double *y, const double *z,
+                 const int *l, const int *m, const int *n, const double *coef, const double *alpha, int i, int j,
+                 int k, int ll, int ax, double *out) {
+    orc_basis B = mkbasis(nbf, nprim, istart, x, y, z, l, m, n, coef, alpha);
+    deri_bf(&B, i, j, k, ll, ax, out);
+}

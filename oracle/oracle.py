@@ -62,6 +62,8 @@ def lib():
         L.orc_grad_2e.restype = None
         L.orc_grad_1e.argtypes = [i] + [p] * 10 + [p, i, p, p, p, p, p, p, p]
         L.orc_grad_1e.restype = None
+        L.orc_deri_bf.argtypes = [i] + [p] * 10 + [i, i, i, i, i, p]
+        L.orc_deri_bf.restype = None
         L.orc_eri_quartets.argtypes = [i] + [p] * 10 + [i, p, p, i]
         L.orc_eri_quartets.restype = d
         _lib = L
@@ -233,6 +235,16 @@ def intindex(i, j, k, l):
 
 def eri(B, i, j, k, l):
     return lib().orc_eri_bf(*B.cargs(), i, j, k, l)
+
+
+def deri(B, i, j, k, l):
+    """d(ij|kl)/dR for the centres of i, j, k, l: array [4 centres, 3 axes]."""
+    out = np.zeros((3, 4))
+    for ax in range(3):
+        o = np.zeros(4)
+        lib().orc_deri_bf(*B.cargs(), i, j, k, l, ax, _ptr(o))
+        out[ax] = o
+    return out.T
 
 
 def eri_quartets_sum(B, first, nfun, nthreads=0):

@@ -5,7 +5,7 @@
 #include <cstring>
 #include <vector>
 
-#include "../../hfsolver_amd/csrc/eri_rys.h"
+#include "../../hfsolver_amd/csrc/grad_rys.h"
 
 #define RYS_TABLE_QUALIFIER static const
 namespace chk_tables {
@@ -20,7 +20,8 @@ RysTables tabs() {
 }
 
 void make_pair(int la, int lb, const double* A, const double* B, int na, const double* aa, const double* ca, int nb,
-               const double* ab, const double* cb, ShellPair& sp, std::vector<PrimPair>& prims) {
+               const double* ab, const double* cb, ShellPair& sp, std::vector<PrimPair>& prims,
+               std::vector<PrimExp>* pexp = nullptr) {
     sp.A = 0;
     sp.B = 1;
     sp.fa = sp.fb = 0;
@@ -47,6 +48,7 @@ void make_pair(int la, int lb, const double* A, const double* B, int na, const d
             pp.PAz = pp.Pz - A[2];
             pp.K = ca[i] * cb[j] * std::exp(-a * b / p * r2);
             prims.push_back(pp);
+            if (pexp) pexp->push_back(PrimExp{a, b});
         }
     sp.np = (int)prims.size() - sp.p0;
 }
@@ -71,6 +73,35 @@ void run(const ShellPair& bra, const ShellPair& ket, const PrimPair* prims, doub
     }
 }
 
+template <int LA, int LB, int LC, int LD, typename R>
+void run_grad(const ShellPair& bra, const ShellPair& ket, const PrimPair* prims, const PrimExp* pexp,
+              const double* gam_full, double* g) {
+    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int split = 0; split < nsplit(LA, LB, LC, LD); split++) {
+        const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
+        const int sc = SPC ? split % Q::NC : 0;
+        R gam[Q::NOUT];
+        for (int a = 0; a < Q::NAt; a++)
+            for (int b = 0; b < Q::NB; b++)
+                for (int c = 0; c < Q::NCt; c++)
+                    for (int d = 0; d < Q::ND; d++) {
+                        const int ga = SPA ? sa : a, gc = SPC ? sc : c;
+                        gam[((a * Q::NB + b) * Q::NCt + c) * Q::ND + d] =
+                            R(gam_full[((ga * Q::NB + b) * Q::NC + gc) * Q::ND + d]);
+                    }
+        grad_quartet<LA, LB, LC, LD, SPA, SPC, R>(bra, ket, prims, pexp, sa, sc, tabs(), gam, acc);
+    }
+    for (int k = 0; k < 9; k++) g[k] = acc[k];
+}
+
+template <int X, int Y, typename R>
+void run_grad_combo(const ShellPair& bra, const ShellPair& ket, const PrimPair* prims, const PrimExp* pexp,
+                    const double* gam, double* g) {
+    run_grad<pair_class_la(X), pair_class_lb(X), pair_class_la(Y), pair_class_lb(Y), R>(bra, ket, prims, pexp, gam, g);
+}
+
 template <int X, int Y, typename R>
 void run_combo(const ShellPair& bra, const ShellPair& ket, const PrimPair* prims, double* out) {
     run<pair_class_la(X), pair_class_lb(X), pair_class_la(Y), pair_class_lb(Y), R>(bra, ket, prims, out);
@@ -82,6 +113,14 @@ typedef void (*fn)(const ShellPair&, const ShellPair&, const PrimPair*, double*)
     {run_combo<X, 0, R>, X >= 1 ? run_combo<X, (X >= 1 ? 1 : 0), R> : nullptr,                          \
      X >= 2 ? run_combo<X, (X >= 2 ? 2 : 0), R> : nullptr, X >= 3 ? run_combo<X, (X >= 3 ? 3 : 0), R> : nullptr, \
      X >= 4 ? run_combo<X, (X >= 4 ? 4 : 0), R> : nullptr, X >= 5 ? run_combo<X, (X >= 5 ? 5 : 0), R> : nullptr}
+
+typedef void (*gfn)(const ShellPair&, const ShellPair&, const PrimPair*, const PrimExp*, const double*, double*);
+#define GROW(X, R)                                                                                             \
+    {run_grad_combo<X, 0, R>, X >= 1 ? run_grad_combo<X, (X >= 1 ? 1 : 0), R> : nullptr,                       \
+     X >= 2 ? run_grad_combo<X, (X >= 2 ? 2 : 0), R> : nullptr, X >= 3 ? run_grad_combo<X, (X >= 3 ? 3 : 0), R> : nullptr, \
+     X >= 4 ? run_grad_combo<X, (X >= 4 ? 4 : 0), R> : nullptr, X >= 5 ? run_grad_combo<X, (X >= 5 ? 5 : 0), R> : nullptr}
+const gfn G64[6][6] = {GROW(0, double), GROW(1, double), GROW(2, double), GROW(3, double), GROW(4, double), GROW(5, double)};
+const gfn G32[6][6] = {GROW(0, float), GROW(1, float), GROW(2, float), GROW(3, float), GROW(4, float), GROW(5, float)};
 
 const fn T64[6][6] = {ROW(0, double), ROW(1, double), ROW(2, double), ROW(3, double), ROW(4, double), ROW(5, double)};
 const fn T32[6][6] = {ROW(0, float), ROW(1, float), ROW(2, float), ROW(3, float), ROW(4, float), ROW(5, float)};
@@ -104,6 +143,23 @@ int chk_eri_quartet(int la, int lb, int lc, int ld, const double* A, const doubl
     make_pair(la, lb, A, B, na, aa, ca, nb, ab, cb, bra, prims);
     make_pair(lc, ld, C, D, nc, ac, cc, nd, ad, cd, ket, prims);
     (fp32 ? T32 : T64)[X][Y](bra, ket, prims.data(), out);
+    return 0;
+}
+
+// g[9] = sum_abcd gam_abcd d(ab|cd)/d(A,B,C) (gam full block, d fastest)
+int chk_grad_quartet(int la, int lb, int lc, int ld, const double* A, const double* B, const double* C,
+                     const double* D, int na, const double* aa, const double* ca, int nb, const double* ab,
+                     const double* cb, int nc, const double* ac, const double* cc, int nd, const double* ad,
+                     const double* cd, const double* gam, int fp32, double* g) {
+    if (la < lb || lc < ld || la > 2 || lc > 2) return 1;
+    const int X = pair_class(la, lb), Y = pair_class(lc, ld);
+    if (X < Y) return 2;
+    std::vector<PrimPair> prims;
+    std::vector<PrimExp> pexp;
+    ShellPair bra, ket;
+    make_pair(la, lb, A, B, na, aa, ca, nb, ab, cb, bra, prims, &pexp);
+    make_pair(lc, ld, C, D, nc, ac, cc, nd, ad, cd, ket, prims, &pexp);
+    (fp32 ? G32 : G64)[X][Y](bra, ket, prims.data(), pexp.data(), gam, g);
     return 0;
 }
 

@@ -1,0 +1,45 @@
+"""The SCF driver (hfsolver_amd/scf.py) with the oracle's G(P) on the host:
+reference semantics reproduce the compiled reference's SCF exactly, and the
+DIIS option converges to the same state (regression: the P = 0 start must not
+enter the DIIS subspace, its commutator error is zero)."""
+import numpy as np
+
+from hfsolver_amd.scf import doscf, nuclear_energy, nuclear_gradient
+from oracle import oracle as O
+
+
+def _setup(g):
+    B = O.Basis(g["atno"], g["xyz"])
+    S, T, V = O.one_electron(B)
+    return B, S, T + V
+
+
+def test_reference_semantics(golden):
+    g = golden("h2o_config0")
+    B, S, H = _setup(g)
+    r = doscf(H, S, 5, lambda P: O.formG(B, P), 40, 1.0, 1e-10, 1e-8)
+    assert r["iterations"] == len(g["E_iter"])
+    assert np.abs(np.array(r["E_iter"]) - g["E_iter"]).max() < 1e-9
+    assert abs(r["E0"] + nuclear_energy(g["atno"], g["xyz"]) - g["Etot"]) < 1e-9
+
+
+def test_diis_converges_to_same_state(golden):
+    g = golden("water2_seed2")
+    B, S, H = _setup(g)
+    r = doscf(H, S, 10, lambda P: O.formG(B, P), 60, 1.0, 1e-10, 1e-8, diis=True)
+    assert r["iterations"] < 30
+    assert abs(r["E0"] + nuclear_energy(g["atno"], g["xyz"]) - g["Etot"]) < 1e-8
+
+
+def test_nuclear_gradient_finite_difference():
+    atno = [8, 1, 1]
+    xyz = np.array([[0.0, 0.0, 0.0], [1.809, 0.0, 0.0], [-0.453549, 1.751221, 0.0]])
+    g = nuclear_gradient(atno, xyz)
+    h = 1e-5
+    for a in range(3):
+        for d in range(3):
+            xp, xm = xyz.copy(), xyz.copy()
+            xp[a, d] += h
+            xm[a, d] -= h
+            fd = (nuclear_energy(atno, xp) - nuclear_energy(atno, xm)) / (2 * h)
+            assert abs(fd - g[a, d]) < 1e-8
