@@ -96,6 +96,12 @@ struct hfx_engine {
         int chunk[2] = {0, 0};
         int* d_blk_row[2] = {nullptr, nullptr};
         int* d_blk_k0[2] = {nullptr, nullptr};
+        int R = 1;              // bra rows per row-mode block
+        // one-row blocks of the gradient kernel
+        int gnblk[2] = {0, 0};
+        int gchunk[2] = {0, 0};
+        int* d_gblk_row[2] = {nullptr, nullptr};
+        int* d_gblk_k0[2] = {nullptr, nullptr};
     };
     std::vector<Combo> combos;
     double npairs_sig = 0;
@@ -463,36 +469,69 @@ int build_lists(hfx_t* h) {
                     cmb.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
                     const size_t need = hfx_row_lds_bytes(cb, ck, cmb.ncols);
                     const int nr = (pair_class_la(cb) + pair_class_lb(cb) + lc + ld) / 2 + 1;
-                    cmb.rowmode = h->opts.kernel_mode == 0 && need > 0 && need <= h->lds_budget;
-                    cmb.boys_lds = nr <= 2 && need + BOYS_SIZE * sizeof(double) <= h->lds_budget;
-                    {   // row blocks: used by row-mode J/K and by the gradient kernel
+                    // rows per block: as many (up to HFX_ROWS_MAX) as the LDS budget allows
+                    // (R > 1 only while R rows stay within 64 KB: two blocks per CU)
+                    const size_t two_per_cu = 64 * 1024;
+                    int R = need > 0 ? (int)std::min<size_t>(HFX_ROWS_MAX, two_per_cu / need) : 0;
+                    if (R == 0 && need > 0 && need <= h->lds_budget) R = 1;
+                    cmb.rowmode = h->opts.kernel_mode == 0 && R >= 1;
+                    cmb.R = std::max(1, R);
+                    const size_t lds_rows = cmb.R * need;
+                    cmb.boys_lds = nr <= 2 && lds_rows + BOYS_SIZE * sizeof(double) <=
+                                                  std::max(lds_rows, (size_t)80 * 1024);
+                    // block lists: (first row, first ket) for groups of rows; rows = cmb.R for
+                    // the row-mode J/K kernel, rows = 1 for the gradient kernel
+                    for (int pass = 0; pass < 2; pass++) {
+                        const int RR = pass == 0 ? cmb.R : 1;
                         for (int prec = 0; prec < 2; prec++) {
-                            // ket chunk per block: whole rows when there are enough of them
                             double tot = 0;
-                            int nrows = 0;
-                            for (int i = 0; i < nb; i++) {
-                                const int len = prec == 0 ? k64[i] : kmax[i] - k64[i];
-                                if (len > 0) {
-                                    tot += len;
-                                    nrows++;
+                            int ngroups = 0;
+                            for (int g0 = 0; g0 < nb; g0 += RR) {
+                                int lo = 1 << 30, hi = 0;
+                                for (int i = g0; i < std::min(nb, g0 + RR); i++) {
+                                    const int b0 = prec == 0 ? 0 : k64[i], b1 = prec == 0 ? k64[i] : kmax[i];
+                                    if (b1 > b0) {
+                                        lo = std::min(lo, b0);
+                                        hi = std::max(hi, b1);
+                                    }
+                                }
+                                if (hi > lo) {
+                                    tot += hi - lo;
+                                    ngroups++;
                                 }
                             }
-                            if (nrows == 0) continue;
+                            if (ngroups == 0) continue;
+                            // ket chunk per block: whole ranges when there are enough groups
                             int chunk = 1 << 20;
                             const double target_blocks = 4096;
-                            if (nrows < target_blocks) chunk = std::max(256, (int)std::ceil(tot / target_blocks / 64.0) * 64);
+                            if (ngroups < target_blocks)
+                                chunk = std::max(256, (int)std::ceil(tot / target_blocks / 256.0) * 256);
                             std::vector<int> br, bk;
-                            for (int i = 0; i < nb; i++) {
-                                const int b0 = prec == 0 ? 0 : k64[i], b1 = prec == 0 ? k64[i] : kmax[i];
-                                for (int k0 = b0; k0 < b1; k0 += chunk) {
-                                    br.push_back(i);
+                            for (int g0 = 0; g0 < nb; g0 += RR) {
+                                int lo = 1 << 30, hi = 0;
+                                for (int i = g0; i < std::min(nb, g0 + RR); i++) {
+                                    const int b0 = prec == 0 ? 0 : k64[i], b1 = prec == 0 ? k64[i] : kmax[i];
+                                    if (b1 > b0) {
+                                        lo = std::min(lo, b0);
+                                        hi = std::max(hi, b1);
+                                    }
+                                }
+                                for (int k0 = lo; k0 < hi; k0 += chunk) {
+                                    br.push_back(g0);
                                     bk.push_back(k0);
                                 }
                             }
-                            cmb.chunk[prec] = chunk;
-                            cmb.nblk[prec] = (int)br.size();
-                            if (upload(h, &cmb.d_blk_row[prec], br, true)) return 1;
-                            if (upload(h, &cmb.d_blk_k0[prec], bk, true)) return 1;
+                            if (pass == 0) {
+                                cmb.chunk[prec] = chunk;
+                                cmb.nblk[prec] = (int)br.size();
+                                if (upload(h, &cmb.d_blk_row[prec], br, true)) return 1;
+                                if (upload(h, &cmb.d_blk_k0[prec], bk, true)) return 1;
+                            } else {
+                                cmb.gchunk[prec] = chunk;
+                                cmb.gnblk[prec] = (int)br.size();
+                                if (upload(h, &cmb.d_gblk_row[prec], br, true)) return 1;
+                                if (upload(h, &cmb.d_gblk_k0[prec], bk, true)) return 1;
+                            }
                         }
                     }
                 }
@@ -508,9 +547,9 @@ int build_lists(hfx_t* h) {
         fprintf(stderr, "hfx: %d shells, %zu pairs (%.0f significant), quartets fp64 %.4g fp32 %.4g\n",
                 (int)h->shells.size(), h->pairs.size(), h->npairs_sig, q64, q32);
         for (auto& c : h->combos)
-            fprintf(stderr, "hfx:   class (%d%d|%d%d) ket np %d %s: quartets %.4g / %.4g, prim quartets %.4g / %.4g\n",
+            fprintf(stderr, "hfx:   class (%d%d|%d%d) ket np %d %s R=%d: quartets %.4g / %.4g, prim quartets %.4g / %.4g\n",
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck),
-                    c.npk, c.rowmode ? "rows" : "tiles", c.nq[0], c.nq[1],
+                    c.npk, c.rowmode ? "rows" : "tiles", c.R, c.nq[0], c.nq[1],
                     c.nprimq[0], c.nprimq[1]);
     }
     return 0;
@@ -787,6 +826,8 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
                 r.blk_row = c.d_blk_row[prec];
                 r.blk_k0 = c.d_blk_k0[prec];
                 r.chunk = c.chunk[prec];
+                r.R = c.R;
+                r.nbra = a.nbra;
                 r.ncols = c.ncols;
                 r.offd = c.offd;
                 const int lc = pair_class_la(c.ck), ld = pair_class_lb(c.ck);
@@ -949,13 +990,13 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         a.bra_list = h->d_cls_list[c.cb];
         a.ket_list = c.d_ket;
         for (int prec = 0; prec < 2; prec++) {
-            if (c.nblk[prec] == 0) continue;
+            if (c.gnblk[prec] == 0) continue;
             a.kbeg = c.d_kbeg[prec];
             a.kend = c.d_kend[prec];
-            a.blk_row = c.d_blk_row[prec];
-            a.blk_k0 = c.d_blk_k0[prec];
-            a.chunk = c.chunk[prec];
-            HIPCHK(h, hfx_launch_grad(c.cb, c.ck, prec == 1, a, c.nblk[prec], h->stream));
+            a.blk_row = c.d_gblk_row[prec];
+            a.blk_k0 = c.d_gblk_k0[prec];
+            a.chunk = c.gchunk[prec];
+            HIPCHK(h, hfx_launch_grad(c.cb, c.ck, prec == 1, a, c.gnblk[prec], h->stream));
         }
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));

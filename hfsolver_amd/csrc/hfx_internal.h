@@ -29,14 +29,14 @@ struct JKArgs {
     int n;
 };
 
-// Arguments of one row-mode J/K launch: block b owns bra row blk_row[b] and
-// the ket columns [blk_k0[b], blk_k0[b] + chunk) of that row (intersected
-// with [kbeg, kend) of the row). The K contributions of the row land in the
-// rows of shells A and B, accumulated in LDS over all the block's kets and
-// flushed once: LDS columns are the functions of the L_C shells (then, when
-// L_D != L_C, those of the L_D shells from column offd on); bf_c / bf_d map
-// columns back to basis functions. J_ab is reduced over the block; only J_cd
-// goes to HBM per quartet.
+// Arguments of one row-mode J/K launch: block b owns bra rows
+// blk_row[b] .. blk_row[b] + R - 1 and the ket columns [blk_k0[b], blk_k0[b] +
+// chunk) (intersected with [kbeg, kend) of each row). The K contributions of
+// each row land in the rows of its shells A and B, accumulated in LDS over all
+// the block's kets and flushed once: LDS columns are the functions of the L_C
+// shells (then, when L_D != L_C, those of the L_D shells from column offd on);
+// bf_c / bf_d map columns back to basis functions. J_ab is reduced over the
+// block; J_cd of each ket is summed over the R rows before one flush.
 struct RowArgs {
     const ShellPair* pairs;
     const PrimPair* prims;
@@ -44,9 +44,11 @@ struct RowArgs {
     const int* ket_list;
     const int* kbeg;
     const int* kend;
-    const int* blk_row;
+    const int* blk_row;   // first of the block's R consecutive bra rows
     const int* blk_k0;
     int chunk;
+    int R;                // bra rows per block (1..HFX_ROWS_MAX): J_cd of a ket is summed over them
+    int nbra;
     int ncols;
     int offd;
     const int* bf_c;
@@ -94,6 +96,7 @@ struct BlockArgs {
     double* out;
 };
 
+static constexpr int HFX_ROWS_MAX = 4;   // bra rows per row-mode block
 static constexpr int HFX_TILE_ROWS = 4;   // one bra row per wavefront
 static constexpr int HFX_TILE_COLS = 64;
 

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 #endif
 
 #if HFX_PART == 2 || HFX_PART == 3
-// Row mode: one block = one bra shell pair x a chunk of its kets (see RowArgs).
+// Row mode: one block = R bra shell pairs x a chunk of their kets (see RowArgs).
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
     constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
@@ -124,95 +124,131 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
     extern __shared__ double smem[];
     const int ncols = a.ncols;
-    const int nkl = (NAt + NB) * ncols;
-    double* Kl = smem;
-    double* Jl = smem + nkl;
+    const int nrow = NAt + NB;                 // LDS K rows per bra row
+    const int row0 = a.blk_row[blockIdx.x];
+    const int nr = min(a.R, a.nbra - row0);    // bra rows of this block
+    const int nkl = a.R * nrow * ncols;
+    double* Kl = smem;                         // [R][NAt + NB][ncols]
+    double* Jl = smem + nkl;                   // [R][NAt][NB]
     const double* boys = hfx_dev::BOYS_TAB;
     if constexpr (Q::NR <= 2) {
         if (a.boys_lds) {  // Boys table staged after the accumulators
-            double* bl = smem + nkl + NAt * NB;
+            double* bl = smem + nkl + a.R * NAt * NB;
             for (int k = threadIdx.x; k < BOYS_SIZE; k += blockDim.x) bl[k] = hfx_dev::BOYS_TAB[k];
             boys = bl;
         }
     }
-    for (int k = threadIdx.x; k < nkl + NAt * NB; k += blockDim.x) smem[k] = 0.0;
+    for (int k = threadIdx.x; k < nkl + a.R * NAt * NB; k += blockDim.x) smem[k] = 0.0;
     __syncthreads();
-    const int row = a.blk_row[blockIdx.x];
     const int split = blockIdx.y;
     const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
     const int sc = SPC ? split % Q::NC : 0;
-    const int bp = a.bra_list[row];
-    const ShellPair bra = a.pairs[bp];
+    const int n = a.n;
+    // ket range of the block: union of its rows' ranges within the chunk
     const int k0 = a.blk_k0[blockIdx.x];
-    const int j0 = max(k0, a.kbeg[row]);
-    const int j1 = min(k0 + a.chunk, a.kend[row]);
-    const int fa = bra.fa + sa, fb = bra.fb, n = a.n;
-    double Jab[NAt][NB];
-#pragma unroll
-    for (int i = 0; i < NAt; i++)
-#pragma unroll
-        for (int j = 0; j < NB; j++) Jab[i][j] = 0.0;
-    const double sbra = bra.A == bra.B ? 0.5 : 1.0;
-    for (int col = j0 + (int)threadIdx.x; col < j1; col += (int)blockDim.x) {
-        const int kp = a.ket_list[col];
-        const ShellPair ket = a.pairs[kp];
-        double s = sbra;
-        if (ket.A == ket.B) s *= 0.5;
-        if (bp == kp) s *= 0.5;
-        R eri[Q::NOUT];
-        eri_quartet<LA, LB, LC, LD, SPA, SPC, R>(bra, ket, a.prims, sa, sc, dev_tables(boys), eri);
-        JKAcc<LA, LB, LC, LD, SPA, SPC, R> acc;
-        jk_zero(acc);
-        jk_accumulate<LA, LB, LC, LD, SPA, SPC, R>(eri, fa, fb, ket.fa + sc, ket.fb, a.D, n, acc);
-        const double s2 = 2.0 * s;
-#pragma unroll
-        for (int i = 0; i < NAt; i++)
-#pragma unroll
-            for (int j = 0; j < NB; j++) Jab[i][j] += s2 * double(acc.Jab[i][j]);
-        const int fc = ket.fa + sc, fd = ket.fb;
+    int lo = 1 << 30, hi = 0;
+    for (int r = 0; r < nr; r++) {
+        lo = min(lo, a.kbeg[row0 + r]);
+        hi = max(hi, a.kend[row0 + r]);
+    }
+    const int j0 = max(k0, lo);
+    const int j1 = min(k0 + a.chunk, hi);
+    const bool lead = (threadIdx.x % 64) == 0;
+    // uniform trip count: every lane of a wavefront reaches the reductions
+    for (int base = j0; base < j1; base += (int)blockDim.x) {
+        const int col = base + (int)threadIdx.x;
+        const bool incol = col < j1;
+        int kp = 0;
+        ShellPair ket;
+        if (incol) {
+            kp = a.ket_list[col];
+            ket = a.pairs[kp];
+        }
+        R Jcd[NCt][ND];
 #pragma unroll
         for (int c = 0; c < NCt; c++)
 #pragma unroll
-            for (int d = 0; d < ND; d++)
-                unsafeAtomicAdd(&a.J[(size_t)(fc + c) * n + fd + d], s2 * double(acc.Jcd[c][d]));
-        const int cc = ket.cpa + sc, cd = a.offd + ket.cpb;
+            for (int d = 0; d < ND; d++) Jcd[c][d] = R(0);
+        for (int r = 0; r < nr; r++) {
+            const int row = row0 + r;
+            const int bp = a.bra_list[row];
+            const ShellPair bra = a.pairs[bp];
+            const bool act = incol && col >= a.kbeg[row] && col < a.kend[row];
+            const int fa = bra.fa + sa, fb = bra.fb;
+            JKAcc<LA, LB, LC, LD, SPA, SPC, R> acc;
+            jk_zero(acc);
+            double s = 0.0;
+            if (act) {
+                s = bra.A == bra.B ? 0.5 : 1.0;
+                if (ket.A == ket.B) s *= 0.5;
+                if (bp == kp) s *= 0.5;
+                R eri[Q::NOUT];
+                eri_quartet<LA, LB, LC, LD, SPA, SPC, R>(bra, ket, a.prims, sa, sc, dev_tables(boys), eri);
+                jk_accumulate<LA, LB, LC, LD, SPA, SPC, R>(eri, fa, fb, ket.fa + sc, ket.fb, a.D, n, acc);
+                const R rs = R(s);
 #pragma unroll
-        for (int i = 0; i < NAt; i++) {
+                for (int c = 0; c < NCt; c++)
 #pragma unroll
-            for (int c = 0; c < NCt; c++) atomicAdd(&Kl[i * ncols + cc + c], s * double(acc.Kac[i][c]));
+                    for (int d = 0; d < ND; d++) Jcd[c][d] += rs * acc.Jcd[c][d];
+                const int cc = ket.cpa + sc, cd = a.offd + ket.cpb;
+                double* Kr = Kl + r * nrow * ncols;
 #pragma unroll
-            for (int d = 0; d < ND; d++) atomicAdd(&Kl[i * ncols + cd + d], s * double(acc.Kad[i][d]));
+                for (int i = 0; i < NAt; i++) {
+#pragma unroll
+                    for (int c = 0; c < NCt; c++) atomicAdd(&Kr[i * ncols + cc + c], s * double(acc.Kac[i][c]));
+#pragma unroll
+                    for (int d = 0; d < ND; d++) atomicAdd(&Kr[i * ncols + cd + d], s * double(acc.Kad[i][d]));
+                }
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+#pragma unroll
+                    for (int c = 0; c < NCt; c++)
+                        atomicAdd(&Kr[(NAt + b) * ncols + cc + c], s * double(acc.Kbc[b][c]));
+#pragma unroll
+                    for (int d = 0; d < ND; d++)
+                        atomicAdd(&Kr[(NAt + b) * ncols + cd + d], s * double(acc.Kbd[b][d]));
+                }
+            }
+            // J_ab of row r: reduce over the wavefront, one LDS add per wavefront
+#pragma unroll
+            for (int i = 0; i < NAt; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const double v = wave_sum(2.0 * s * double(acc.Jab[i][j]));
+                    if (lead && v != 0.0) atomicAdd(&Jl[(r * NAt + i) * NB + j], v);
+                }
         }
+        if (incol) {
+            const int fc = ket.fa + sc, fd = ket.fb;
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
+            for (int c = 0; c < NCt; c++)
 #pragma unroll
-            for (int c = 0; c < NCt; c++) atomicAdd(&Kl[(NAt + b) * ncols + cc + c], s * double(acc.Kbc[b][c]));
-#pragma unroll
-            for (int d = 0; d < ND; d++) atomicAdd(&Kl[(NAt + b) * ncols + cd + d], s * double(acc.Kbd[b][d]));
+                for (int d = 0; d < ND; d++) {
+                    const double v = 2.0 * double(Jcd[c][d]);
+                    if (v != 0.0) unsafeAtomicAdd(&a.J[(size_t)(fc + c) * n + fd + d], v);
+                }
         }
     }
-    // J_ab: reduce over the wavefront, then over the block in LDS
-#pragma unroll
-    for (int i = 0; i < NAt; i++)
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-            const double v = wave_sum(Jab[i][j]);
-            if ((threadIdx.x % 64) == 0 && v != 0.0) atomicAdd(&Jl[i * NB + j], v);
-        }
     __syncthreads();
-    for (int k = threadIdx.x; k < nkl; k += blockDim.x) {
+    for (int k = threadIdx.x; k < nr * nrow * ncols; k += blockDim.x) {
         const double v = Kl[k];
         if (v == 0.0) continue;
-        const int r = k / ncols, col = k - r * ncols;
-        const int rbf = r < NAt ? fa + r : fb + (r - NAt);
+        const int r = k / (nrow * ncols), rem = k - r * nrow * ncols;
+        const int i = rem / ncols, col = rem - i * ncols;
+        const ShellPair bra = a.pairs[a.bra_list[row0 + r]];
+        const int rbf = i < NAt ? bra.fa + sa + i : bra.fb + (i - NAt);
         const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[col - a.offd] : a.bf_c[col];
         unsafeAtomicAdd(&a.K[(size_t)rbf * n + cbf], v);
     }
-    for (int k = threadIdx.x; k < NAt * NB; k += blockDim.x) {
+    for (int k = threadIdx.x; k < nr * NAt * NB; k += blockDim.x) {
         const double v = Jl[k];
-        if (v != 0.0) unsafeAtomicAdd(&a.J[(size_t)(fa + k / NB) * n + fb + k % NB], v);
+        if (v == 0.0) continue;
+        const int r = k / (NAt * NB), ij = k - r * NAt * NB;
+        const ShellPair bra = a.pairs[a.bra_list[row0 + r]];
+        unsafeAtomicAdd(&a.J[(size_t)(bra.fa + sa + ij / NB) * n + bra.fb + ij % NB], v);
     }
 }
+
 #endif
 
 #if HFX_PART == 4 || HFX_PART == 5
@@ -393,7 +429,7 @@ typedef HFX_REAL_T RowReal;
 template <int X, int Y>
 hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
     constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
-    const size_t lds = ((size_t)row_nab<HFX_CLS(X), HFX_CLS(Y)>() * a.ncols + NAt * ncart(pair_class_lb(X)) +
+    const size_t lds = ((size_t)a.R * (row_nab<HFX_CLS(X), HFX_CLS(Y)>() * a.ncols + NAt * ncart(pair_class_lb(X))) +
                         (a.boys_lds ? BOYS_SIZE : 0)) * sizeof(double);
     static bool attr_set = false;
     if (!attr_set) {
