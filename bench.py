@@ -25,6 +25,22 @@ PEAK_FP64 = 78.6e12   # MI355X vector / matrix FP64, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X vector FP32 (packed), FLOP/s
 
 
+TRAFFIC_FILE = "profiles/jk_fock_64w.md"
+
+
+def measured_traffic():
+    """HBM bytes of one FP64 build of the 64-water workload (FETCH_SIZE + WRITE_SIZE
+    of its J/K dispatches, scaled by a plain-copy calibration), as recorded by the
+    rocprofv3 --pmc runs summarised in TRAFFIC_FILE; None when absent."""
+    try:
+        for line in open(os.path.join(ROOT, TRAFFIC_FILE)):
+            if line.startswith("traffic_bytes_per_fp64_build:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -167,6 +183,9 @@ def main():
     # mixed: fraction of the time-weighted peak (FP64 part at 78.6, FP32 part at 157.3 TFLOP/s)
     frac_mx = (fmx64 / PEAK_FP64 + fmx32 / PEAK_FP32) / jkmx
     value = args.steps / tmx
+    traffic = None
+    if args.nwater == 64 and args.seed == 64 and args.density == "core":
+        traffic = measured_traffic()
     out = {
         "metric": "J+K Fock builds/s, 64 H2O 6-31G**: mixed-FP32 vs FP64 GPU, |dE|<1e-6 Eh",
         "value": value,
@@ -189,8 +208,10 @@ def main():
         "mixed_max_dF": dF,
         "E_elec_fp64_at_density": E64,
         "roofline": {"bound": "fp64-valu", "achieved": roof64 / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": roof64 / PEAK_FP64, "traffic": None,
-                     "kernel": "jk_kernel (all class launches of one FP64 build)",
+                     "unit": "TFLOP/s", "frac": roof64 / PEAK_FP64, "traffic": traffic,
+                     "kernel": "jk_row_kernel/jk_kernel (all class launches of one FP64 build)",
+                     "jk_seconds_fp64": float(jk64), "jk_seconds_mixed": float(jkmx),
+                     "flops_fp64": float(f64), "traffic_source": TRAFFIC_FILE if traffic else None,
                      "mixed_frac_of_weighted_peak": frac_mx},
         "cpu_baseline": cpu,
     }
