@@ -137,13 +137,49 @@ HFX_HD HFX_INLINE void rys_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R 
                 }
 }
 
+// Primitive-pair data of a ket as the quartet loop reads it: exponent sum q,
+// 1/q, centre Q, Q - C, and the prefactor K_cd.
+struct KetPrim {
+    double p, ip, Px, Py, Pz, QCx, QCy, QCz, K;
+};
+
+// Ket primitives stored as PrimPair records (the ket PrimPair keeps Q - C in
+// its PA fields).
+struct KetPrimsAoS {
+    const PrimPair* pp;
+    int np;
+    HFX_HD HFX_INLINE int count() const { return np; }
+    HFX_HD HFX_INLINE KetPrim get(int k) const {
+        const PrimPair& q = pp[k];
+        return KetPrim{q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, q.K};
+    }
+};
+
+// Ket primitives of a ket list stored field-major, so that consecutive kets
+// (consecutive lanes) read consecutive addresses: field f of primitive k of
+// ket j at base[(k * KETPRIM_FIELDS + f) * stride + j], base already offset by j.
+struct KetPrimsSoA {
+    const double* base;
+    int np;
+    int stride;
+    HFX_HD HFX_INLINE int count() const { return np; }
+    HFX_HD HFX_INLINE KetPrim get(int k) const {
+        const double* b = base + (size_t)k * KETPRIM_FIELDS * stride;
+        return KetPrim{b[0], b[stride], b[2 * stride], b[3 * stride], b[4 * stride],
+                       b[5 * stride], b[6 * stride], b[7 * stride], b[8 * stride]};
+    }
+};
+
 // Contracted integrals of the shell quartet (bra | ket), slice (sa, sc):
 // eri[((a*NB + b)*NCt + c)*ND + d] for the thread's a- and c-components.
-// bra/ket: shell pairs with l_A = LA, l_B = LB, l_C = LC, l_D = LD.
-template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
-HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, const PrimPair* __restrict__ prims,
-                                   int sa, int sc, const RysTables& tab,
-                                   R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT]) {
+// bra: shell pair with l_A = LA, l_B = LB (primitive pairs prims[bra.p0 ..]);
+// ket: l_C = LC, l_D = LD, C - D = (cdx, cdy, cdz), primitives from kp.
+// Ket primitives form the outer loop (one per-lane load each); the bra's are
+// the same for the whole wavefront.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R, class KP>
+HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const PrimPair* __restrict__ prims, double cdx_,
+                                     double cdy_, double cdz_, const KP& kp, int sa, int sc, const RysTables& tab,
+                                     R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT]) {
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NR = Q::NR;
 #pragma unroll
@@ -169,11 +205,12 @@ HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, c
             }
     }
     const R abx = R(bra.ABx), aby = R(bra.ABy), abz = R(bra.ABz);
-    const R cdx = R(ket.ABx), cdy = R(ket.ABy), cdz = R(ket.ABz);
-    for (int i1 = 0; i1 < bra.np; i1++) {
-        const PrimPair P = prims[bra.p0 + i1];
-        for (int k1 = 0; k1 < ket.np; k1++) {
-            const PrimPair Qp = prims[ket.p0 + k1];
+    const R cdx = R(cdx_), cdy = R(cdy_), cdz = R(cdz_);
+    const int nkp = kp.count();
+    for (int k1 = 0; k1 < nkp; k1++) {
+        const KetPrim Qp = kp.get(k1);
+        for (int i1 = 0; i1 < bra.np; i1++) {
+            const PrimPair P = prims[bra.p0 + i1];
             const double p = P.p, q = Qp.p, pq = p + q;
             const double ip = P.ip, iq = Qp.ip;
             const double isq = hfx_rsqrt(pq);
@@ -187,7 +224,6 @@ HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, c
                 rys_roots_small<NR>(T, u, w, tab.boys);
             else
                 rys_roots<NR>(T, u, w, tab);
-            // the ket PrimPair stores Q - C in its PA fields
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const double ur = u[r];
@@ -199,11 +235,11 @@ HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, c
                 R Ix[Q::XA][LB + 1][Q::XC][LD + 1];
                 R Iy[Q::XA][LB + 1][Q::XC][LD + 1];
                 R Iz[Q::XA][LB + 1][Q::XC][LD + 1];
-                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAx - fp * PQx), R(Qp.PAx + fq * PQx), b10, b01, b00, abx,
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAx - fp * PQx), R(Qp.QCx + fq * PQx), b10, b01, b00, abx,
                                                      cdx, R(1), sel_a[0], sel_c[0], Ix);
-                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAy - fp * PQy), R(Qp.PAy + fq * PQy), b10, b01, b00, aby,
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAy - fp * PQy), R(Qp.QCy + fq * PQy), b10, b01, b00, aby,
                                                      cdy, R(1), sel_a[1], sel_c[1], Iy);
-                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAz - fp * PQz), R(Qp.PAz + fq * PQz), b10, b01, b00, abz,
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAz - fp * PQz), R(Qp.QCz + fq * PQz), b10, b01, b00, abz,
                                                      cdz, R(w[r] * pref), sel_a[2], sel_c[2], Iz);
 #pragma unroll
                 for (int a = 0; a < Q::NAt; a++) {
@@ -229,6 +265,15 @@ HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, c
             }
         }
     }
+}
+
+// The same for a ket given as a shell pair with PrimPair records.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void eri_quartet(const ShellPair& bra, const ShellPair& ket, const PrimPair* __restrict__ prims,
+                                   int sa, int sc, const RysTables& tab,
+                                   R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT]) {
+    eri_quartet_k<LA, LB, LC, LD, SPA, SPC, R>(bra, prims, ket.ABx, ket.ABy, ket.ABz,
+                                               KetPrimsAoS{prims + ket.p0, ket.np}, sa, sc, tab, eri);
 }
 
 // J/K digestion of one slice of a quartet: the contributions of one quartet
@@ -271,34 +316,20 @@ HFX_HD HFX_INLINE void jk_zero(JKAcc<LA, LB, LC, LD, SPA, SPC, R>& acc) {
         for (int d = 0; d < Q::ND; d++) acc.Jcd[c][d] = R(0);
 }
 
-// fa/fc: first function of the thread's a / c slice (fa + sa when slicing).
+// The six density blocks one quartet slice is contracted with.
 template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
-HFX_HD HFX_INLINE void jk_accumulate(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], int fa, int fb, int fc,
-                                     int fd, const double* __restrict__ D, int n,
-                                     JKAcc<LA, LB, LC, LD, SPA, SPC, R>& acc) {
+struct JKDens {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    R ab[Q::NAt][Q::NB], cd[Q::NCt][Q::ND];
+    R ac[Q::NAt][Q::NCt], ad[Q::NAt][Q::ND], bc[Q::NB][Q::NCt], bd[Q::NB][Q::ND];
+};
+
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void jk_contract(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT],
+                                   const JKDens<LA, LB, LC, LD, SPA, SPC, R>& D,
+                                   JKAcc<LA, LB, LC, LD, SPA, SPC, R>& acc) {
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
-    R Dab[NAt][NB], Dcd[NCt][ND], Dbd[NB][ND], Dbc[NB][NCt], Dad[NAt][ND], Dac[NAt][NCt];
-#pragma unroll
-    for (int a = 0; a < NAt; a++) {
-#pragma unroll
-        for (int b = 0; b < NB; b++) Dab[a][b] = R(D[(size_t)(fa + a) * n + fb + b]);
-#pragma unroll
-        for (int d = 0; d < ND; d++) Dad[a][d] = R(D[(size_t)(fa + a) * n + fd + d]);
-#pragma unroll
-        for (int c = 0; c < NCt; c++) Dac[a][c] = R(D[(size_t)(fa + a) * n + fc + c]);
-    }
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-#pragma unroll
-        for (int d = 0; d < ND; d++) Dbd[b][d] = R(D[(size_t)(fb + b) * n + fd + d]);
-#pragma unroll
-        for (int c = 0; c < NCt; c++) Dbc[b][c] = R(D[(size_t)(fb + b) * n + fc + c]);
-    }
-#pragma unroll
-    for (int c = 0; c < NCt; c++)
-#pragma unroll
-        for (int d = 0; d < ND; d++) Dcd[c][d] = R(D[(size_t)(fc + c) * n + fd + d]);
 #pragma unroll
     for (int a = 0; a < NAt; a++)
 #pragma unroll
@@ -308,11 +339,42 @@ HFX_HD HFX_INLINE void jk_accumulate(const R (&eri)[QClass<LA, LB, LC, LD, SPA, 
 #pragma unroll
                 for (int d = 0; d < ND; d++) {
                     const R v = eri[((a * NB + b) * NCt + c) * ND + d];
-                    acc.Jab[a][b] = v * Dcd[c][d] + acc.Jab[a][b];
-                    acc.Jcd[c][d] = v * Dab[a][b] + acc.Jcd[c][d];
-                    acc.Kac[a][c] = v * Dbd[b][d] + acc.Kac[a][c];
-                    acc.Kad[a][d] = v * Dbc[b][c] + acc.Kad[a][d];
-                    acc.Kbc[b][c] = v * Dad[a][d] + acc.Kbc[b][c];
-                    acc.Kbd[b][d] = v * Dac[a][c] + acc.Kbd[b][d];
+                    acc.Jab[a][b] = v * D.cd[c][d] + acc.Jab[a][b];
+                    acc.Jcd[c][d] = v * D.ab[a][b] + acc.Jcd[c][d];
+                    acc.Kac[a][c] = v * D.bd[b][d] + acc.Kac[a][c];
+                    acc.Kad[a][d] = v * D.bc[b][c] + acc.Kad[a][d];
+                    acc.Kbc[b][c] = v * D.ad[a][d] + acc.Kbc[b][c];
+                    acc.Kbd[b][d] = v * D.ac[a][c] + acc.Kbd[b][d];
                 }
+}
+
+// fa/fc: first function of the thread's a / c slice (fa + sa when slicing).
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void jk_accumulate(const R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], int fa, int fb, int fc,
+                                     int fd, const double* __restrict__ D, int n,
+                                     JKAcc<LA, LB, LC, LD, SPA, SPC, R>& acc) {
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
+    JKDens<LA, LB, LC, LD, SPA, SPC, R> dn;
+#pragma unroll
+    for (int a = 0; a < NAt; a++) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) dn.ab[a][b] = R(D[(size_t)(fa + a) * n + fb + b]);
+#pragma unroll
+        for (int d = 0; d < ND; d++) dn.ad[a][d] = R(D[(size_t)(fa + a) * n + fd + d]);
+#pragma unroll
+        for (int c = 0; c < NCt; c++) dn.ac[a][c] = R(D[(size_t)(fa + a) * n + fc + c]);
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) dn.bd[b][d] = R(D[(size_t)(fb + b) * n + fd + d]);
+#pragma unroll
+        for (int c = 0; c < NCt; c++) dn.bc[b][c] = R(D[(size_t)(fb + b) * n + fc + c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NCt; c++)
+#pragma unroll
+        for (int d = 0; d < ND; d++) dn.cd[c][d] = R(D[(size_t)(fc + c) * n + fd + d]);
+    jk_contract<LA, LB, LC, LD, SPA, SPC, R>(eri, dn, acc);
 }

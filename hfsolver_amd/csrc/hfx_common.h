@@ -61,6 +61,9 @@ struct PrimPair {
     double pad;
 };
 
+// Fields of a ket primitive pair in a field-major ket table: p, 1/p, P, P - C, K.
+static constexpr int KETPRIM_FIELDS = 9;
+
 // Exponents of a primitive pair (used by the derivative integrals).
 struct PrimExp {
     double aa, ab;

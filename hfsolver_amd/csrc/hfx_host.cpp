@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -88,15 +89,7 @@ struct hfx_engine {
         int* d_kend[2] = {nullptr, nullptr};
         double nq[2] = {0, 0};
         double nprimq[2] = {0, 0};
-        // row mode (K rows of the bra shells accumulated in LDS)
-        bool rowmode = false;
-        int boys_lds = 0;
-        int ncols = 0, offd = 0;
-        int nblk[2] = {0, 0};
-        int chunk[2] = {0, 0};
-        int* d_blk_row[2] = {nullptr, nullptr};
-        int* d_blk_k0[2] = {nullptr, nullptr};
-        int R = 1;              // bra rows per row-mode block
+        bool rowmode = false;   // J/K of (cb, ck) by the row-mode kernel (RowCombo)
         // one-row blocks of the gradient kernel
         int gnblk[2] = {0, 0};
         int gchunk[2] = {0, 0};
@@ -104,6 +97,28 @@ struct hfx_engine {
         int* d_gblk_k0[2] = {nullptr, nullptr};
     };
     std::vector<Combo> combos;
+    // ket tables of every pair class (sublists in ascending primitive-pair count)
+    struct KetTable {
+        KetTab t{};
+        int ncomp = 0, nd = 0;
+    };
+    KetTable ktab[HFX_NPAIRCLASS];
+    // row-mode J/K of one class combination over the whole ket table
+    struct RowCombo {
+        int cb, ck;
+        int R = 1, dlds = 0, boys_lds = 0, ncols = 0, offd = 0;
+        int nblk[2] = {0, 0};
+        int chunk[2] = {0, 0};
+        int* d_kbeg[2] = {nullptr, nullptr};
+        int* d_kend[2] = {nullptr, nullptr};
+        int* d_blk_row[2] = {nullptr, nullptr};
+        int* d_blk_k0[2] = {nullptr, nullptr};
+    };
+    std::vector<RowCombo> rcombos;
+    // tuning of the row-mode blocks (HFX_ROW_RMAX, HFX_ROW_PAIR_KB, HFX_ROW_DLDS)
+    int row_rmax = HFX_ROWS_MAX;
+    size_t row_pair_bytes = 64 * 1024;
+    int row_dlds = 1;
     double npairs_sig = 0;
     // device memory
     ShellPair* d_pairs = nullptr;
@@ -161,6 +176,8 @@ void free_geometry(hfx_t* h) {
     for (void* p : h->geom_allocs) (void)hipFree(p);
     h->geom_allocs.clear();
     h->combos.clear();
+    h->rcombos.clear();
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) h->ktab[c] = hfx_engine::KetTable{};
     h->d_pairs = nullptr;
     h->d_prims = nullptr;
     h->d_pexp = nullptr;
@@ -365,6 +382,68 @@ int compute_schwarz(hfx_t* h) {
     return 0;
 }
 
+// Ket positions per block: whole ranges when there are enough row groups to
+// fill the device, otherwise about 4096 blocks' worth (multiples of 256).
+int block_chunk(double tot, int ngroups) {
+    const double target_blocks = 4096;
+    if (ngroups >= target_blocks) return 1 << 20;
+    return std::max(256, (int)std::ceil(tot / target_blocks / 256.0) * 256);
+}
+
+// Field-major ket table of pair class c from its sublists (see KetTab).
+int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) {
+    hfx_engine::KetTable& T = h->ktab[c];
+    T = hfx_engine::KetTable{};
+    KetTab& t = T.t;
+    const int S = (int)subl.size();
+    t.S = S;
+    if (S == 0 || S > HFX_SUB_MAX) return 0;   // no table: the class is a ket of tile-mode launches only
+    T.nd = ncart(pair_class_lb(c));
+    T.ncomp = ncart(pair_class_la(c)) * T.nd;
+    int nk = 0;
+    long long ps = 0;
+    for (int s = 0; s < S; s++) {
+        t.off[s] = nk;
+        t.np[s] = h->pairs[subl[s][0]].np;
+        t.psoff[s] = ps;
+        nk += (int)subl[s].size();
+        ps += (long long)t.np[s] * KETPRIM_FIELDS * (long long)subl[s].size();
+    }
+    t.off[S] = nk;
+    t.nk = nk;
+    std::vector<KetInfo> info(nk);
+    std::vector<double> cd(3 * (size_t)nk), prim((size_t)ps);
+    for (int s = 0; s < S; s++) {
+        const int nks = (int)subl[s].size();
+        for (int j = 0; j < nks; j++) {
+            const ShellPair& sp = h->pairs[subl[s][j]];
+            const int k = t.off[s] + j;
+            if (sp.cpa >= 32768 || sp.cpb >= 32768) return fail(h, "ket table: too many functions of one angular momentum");
+            info[k] = KetInfo{subl[s][j], sp.fa, sp.fb, sp.cpa | (sp.cpb << 15) | ((sp.A == sp.B ? 1 : 0) << 30)};
+            cd[k] = sp.ABx;
+            cd[(size_t)nk + k] = sp.ABy;
+            cd[2 * (size_t)nk + k] = sp.ABz;
+            for (int i = 0; i < t.np[s]; i++) {
+                const PrimPair& q = h->prims[sp.p0 + i];
+                const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, q.K};
+                for (int fi = 0; fi < KETPRIM_FIELDS; fi++)
+                    prim[(size_t)t.psoff[s] + ((size_t)i * KETPRIM_FIELDS + fi) * nks + j] = f[fi];
+            }
+        }
+    }
+    KetInfo* d_info = nullptr;
+    double *d_cd = nullptr, *d_prim = nullptr;
+    if (upload(h, &d_info, info, true)) return 1;
+    if (upload(h, &d_cd, cd, true)) return 1;
+    if (upload(h, &d_prim, prim, true)) return 1;
+    if (dmalloc(h, (void**)&t.Dcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
+    if (dmalloc(h, (void**)&t.Jcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
+    t.info = d_info;
+    t.cd = d_cd;
+    t.prim = d_prim;
+    return 0;
+}
+
 int build_lists(hfx_t* h) {
     const double thr = h->opts.schwarz_threshold;
     const double thr32 = h->opts.fp32_threshold;
@@ -384,24 +463,44 @@ int build_lists(hfx_t* h) {
             if (upload(h, &h->d_cls_list[c], h->cls_list[c], true)) return 1;
     }
     h->combos.clear();
+    h->rcombos.clear();
     // position of every significant pair in its class list
     std::vector<int> pos(h->pairs.size(), -1);
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
         for (int i = 0; i < (int)h->cls_list[c].size(); i++) pos[h->cls_list[c][i]] = i;
+    // ket tables: sublists of equal primitive-pair count, each still Q-descending
+    std::vector<std::vector<int>> subl[HFX_NPAIRCLASS];
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) {
+        if (h->cls_list[c].empty()) continue;
+        std::vector<int> nps;
+        for (int pid : h->cls_list[c]) nps.push_back(h->pairs[pid].np);
+        std::sort(nps.begin(), nps.end());
+        nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
+        for (int npk : nps) {
+            std::vector<int> kl;
+            for (int pid : h->cls_list[c])
+                if (h->pairs[pid].np == npk) kl.push_back(pid);
+            subl[c].push_back(kl);
+        }
+        if (build_ket_table(h, c, subl[c])) return 1;
+    }
     for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
         for (int ck = 0; ck <= cb; ck++) {
             const std::vector<int>& bl = h->cls_list[cb];
             if (bl.empty() || h->cls_list[ck].empty()) continue;
-            // ket sublists of equal primitive-pair count, each still Q-descending
-            std::vector<int> nps;
-            for (int pid : h->cls_list[ck]) nps.push_back(h->pairs[pid].np);
-            std::sort(nps.begin(), nps.end());
-            nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
-            for (int npk : nps) {
-                std::vector<int> kl;
-                for (int pid : h->cls_list[ck])
-                    if (h->pairs[pid].np == npk) kl.push_back(pid);
-                const int nb = (int)bl.size();
+            const int nb = (int)bl.size();
+            const int S = (int)subl[ck].size();
+            // per row and sublist: [begin, end) of each precision, as ket-table positions
+            std::vector<int> rkb[2], rke[2];
+            for (int prec = 0; prec < 2; prec++) {
+                rkb[prec].assign((size_t)nb * S, 0);
+                rke[prec].assign((size_t)nb * S, 0);
+            }
+            size_t first_combo = h->combos.size();
+            for (int s = 0; s < S; s++) {
+                const std::vector<int>& kl = subl[ck][s];
+                const int npk = h->pairs[kl[0]].np;
+                const int soff = S <= HFX_SUB_MAX ? h->ktab[ck].t.off[s] : 0;
                 const int nk = (int)kl.size();
                 std::vector<double> kq(nk);
                 std::vector<int> kpos(nk);
@@ -440,6 +539,8 @@ int build_lists(hfx_t* h) {
                         const double nqi = std::max(0, ke[i] - kb[i]);
                         cmb.nq[prec] += nqi;
                         cmb.nprimq[prec] += nqi * h->pairs[bl[i]].np * npk;
+                        rkb[prec][(size_t)i * S + s] = soff + kb[i];
+                        rke[prec][(size_t)i * S + s] = soff + std::max(kb[i], ke[i]);
                     }
                     for (int r0 = 0; r0 < nb; r0 += HFX_TILE_ROWS) {
                         int lo = 1 << 30, hi = 0;
@@ -461,82 +562,109 @@ int build_lists(hfx_t* h) {
                         if (upload(h, &cmb.d_kbeg[prec], kb, true)) return 1;
                         if (upload(h, &cmb.d_kend[prec], ke, true)) return 1;
                     }
-                }
-                // row mode when the K rows of the bra shells fit in LDS
-                {
-                    const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
-                    cmb.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
-                    cmb.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
-                    const size_t need = hfx_row_lds_bytes(cb, ck, cmb.ncols);
-                    const int nr = (pair_class_la(cb) + pair_class_lb(cb) + lc + ld) / 2 + 1;
-                    // rows per block: as many (up to HFX_ROWS_MAX) as the LDS budget allows
-                    // (R > 1 only while R rows stay within 64 KB: two blocks per CU)
-                    const size_t two_per_cu = 64 * 1024;
-                    int R = need > 0 ? (int)std::min<size_t>(HFX_ROWS_MAX, two_per_cu / need) : 0;
-                    if (R == 0 && need > 0 && need <= h->lds_budget) R = 1;
-                    cmb.rowmode = h->opts.kernel_mode == 0 && R >= 1;
-                    cmb.R = std::max(1, R);
-                    const size_t lds_rows = cmb.R * need;
-                    cmb.boys_lds = nr <= 2 && lds_rows + BOYS_SIZE * sizeof(double) <=
-                                                  std::max(lds_rows, (size_t)80 * 1024);
-                    // block lists: (first row, first ket) for groups of rows; rows = cmb.R for
-                    // the row-mode J/K kernel, rows = 1 for the gradient kernel
-                    for (int pass = 0; pass < 2; pass++) {
-                        const int RR = pass == 0 ? cmb.R : 1;
-                        for (int prec = 0; prec < 2; prec++) {
-                            double tot = 0;
-                            int ngroups = 0;
-                            for (int g0 = 0; g0 < nb; g0 += RR) {
-                                int lo = 1 << 30, hi = 0;
-                                for (int i = g0; i < std::min(nb, g0 + RR); i++) {
-                                    const int b0 = prec == 0 ? 0 : k64[i], b1 = prec == 0 ? k64[i] : kmax[i];
-                                    if (b1 > b0) {
-                                        lo = std::min(lo, b0);
-                                        hi = std::max(hi, b1);
-                                    }
-                                }
-                                if (hi > lo) {
-                                    tot += hi - lo;
-                                    ngroups++;
-                                }
-                            }
-                            if (ngroups == 0) continue;
-                            // ket chunk per block: whole ranges when there are enough groups
-                            int chunk = 1 << 20;
-                            const double target_blocks = 4096;
-                            if (ngroups < target_blocks)
-                                chunk = std::max(256, (int)std::ceil(tot / target_blocks / 256.0) * 256);
-                            std::vector<int> br, bk;
-                            for (int g0 = 0; g0 < nb; g0 += RR) {
-                                int lo = 1 << 30, hi = 0;
-                                for (int i = g0; i < std::min(nb, g0 + RR); i++) {
-                                    const int b0 = prec == 0 ? 0 : k64[i], b1 = prec == 0 ? k64[i] : kmax[i];
-                                    if (b1 > b0) {
-                                        lo = std::min(lo, b0);
-                                        hi = std::max(hi, b1);
-                                    }
-                                }
-                                for (int k0 = lo; k0 < hi; k0 += chunk) {
-                                    br.push_back(g0);
-                                    bk.push_back(k0);
-                                }
-                            }
-                            if (pass == 0) {
-                                cmb.chunk[prec] = chunk;
-                                cmb.nblk[prec] = (int)br.size();
-                                if (upload(h, &cmb.d_blk_row[prec], br, true)) return 1;
-                                if (upload(h, &cmb.d_blk_k0[prec], bk, true)) return 1;
-                            } else {
-                                cmb.gchunk[prec] = chunk;
-                                cmb.gnblk[prec] = (int)br.size();
-                                if (upload(h, &cmb.d_gblk_row[prec], br, true)) return 1;
-                                if (upload(h, &cmb.d_gblk_k0[prec], bk, true)) return 1;
-                            }
+                    // one-row blocks of the gradient kernel
+                    std::vector<int> br, bk;
+                    double tot = 0;
+                    int ngroups = 0;
+                    for (int i = 0; i < nb; i++)
+                        if (ke[i] > kb[i]) {
+                            tot += ke[i] - kb[i];
+                            ngroups++;
                         }
-                    }
+                    if (ngroups == 0) continue;
+                    const int chunk = block_chunk(tot, ngroups);
+                    for (int i = 0; i < nb; i++)
+                        for (int k0 = kb[i]; k0 < ke[i]; k0 += chunk) {
+                            br.push_back(i);
+                            bk.push_back(k0);
+                        }
+                    cmb.gchunk[prec] = chunk;
+                    cmb.gnblk[prec] = (int)br.size();
+                    if (upload(h, &cmb.d_gblk_row[prec], br, true)) return 1;
+                    if (upload(h, &cmb.d_gblk_k0[prec], bk, true)) return 1;
                 }
                 if (cmb.ntiles[0] + cmb.ntiles[1] > 0) h->combos.push_back(cmb);
             }
+            // row mode when the K rows of the bra shells fit in LDS
+            hfx_engine::RowCombo rc;
+            rc.cb = cb;
+            rc.ck = ck;
+            const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
+            rc.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
+            rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
+            int nrow = 0, nab = 0;
+            hfx_row_shape(cb, ck, &nrow, &nab);
+            const int nr = (pair_class_la(cb) + pair_class_lb(cb) + lc + ld) / 2 + 1;
+            int dlds = h->row_dlds;
+            if (dlds && hfx_row_lds(nrow, nab, rc.ncols, 1, 1, 0, false) > h->lds_budget) dlds = 0;
+            const size_t need = hfx_row_lds(nrow, nab, rc.ncols, 1, dlds, 0, false);
+            // rows per block: as many (up to row_rmax) as stay within row_pair_bytes (two blocks per CU)
+            int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
+            if (R == 0 && need <= h->lds_budget) R = 1;
+            const bool rowmode = h->opts.kernel_mode == 0 && R >= 1 && S <= HFX_SUB_MAX && rc.ncols < 32768;
+            for (size_t ci = first_combo; ci < h->combos.size(); ci++) h->combos[ci].rowmode = rowmode;
+            if (!rowmode) continue;
+            rc.R = R;
+            rc.dlds = dlds;
+            const size_t lds_rows = hfx_row_lds(nrow, nab, rc.ncols, R, dlds, 0, false);
+            rc.boys_lds = nr <= 2 && lds_rows + BOYS_SIZE * sizeof(double) <= std::max(lds_rows, (size_t)80 * 1024);
+            // blocks: (first row, first ket position) for groups of R rows, chunks of the
+            // table positions that meet at least one of the group's ranges
+            for (int prec = 0; prec < 2; prec++) {
+                double tot = 0;
+                int ngroups = 0;
+                for (int g0 = 0; g0 < nb; g0 += R) {
+                    double span = 0;
+                    for (int s = 0; s < S; s++) {
+                        int lo = 1 << 30, hi = 0;
+                        for (int i = g0; i < std::min(nb, g0 + R); i++) {
+                            const size_t x = (size_t)i * S + s;
+                            if (rke[prec][x] > rkb[prec][x]) {
+                                lo = std::min(lo, rkb[prec][x]);
+                                hi = std::max(hi, rke[prec][x]);
+                            }
+                        }
+                        if (hi > lo) span += hi - lo;
+                    }
+                    if (span > 0) {
+                        tot += span;
+                        ngroups++;
+                    }
+                }
+                if (ngroups == 0) continue;
+                const int chunk = block_chunk(tot, ngroups);
+                std::vector<int> br, bk;
+                std::vector<std::pair<int, int>> rng;
+                for (int g0 = 0; g0 < nb; g0 += R) {
+                    rng.clear();
+                    for (int s = 0; s < S; s++) {
+                        int lo = 1 << 30, hi = 0;
+                        for (int i = g0; i < std::min(nb, g0 + R); i++) {
+                            const size_t x = (size_t)i * S + s;
+                            if (rke[prec][x] > rkb[prec][x]) {
+                                lo = std::min(lo, rkb[prec][x]);
+                                hi = std::max(hi, rke[prec][x]);
+                            }
+                        }
+                        if (hi > lo) rng.push_back({lo, hi});
+                    }
+                    if (rng.empty()) continue;
+                    for (int k0 = rng.front().first; k0 < rng.back().second; k0 += chunk) {
+                        bool meets = false;
+                        for (auto& q : rng) meets = meets || (q.first < k0 + chunk && q.second > k0);
+                        if (!meets) continue;
+                        br.push_back(g0);
+                        bk.push_back(k0);
+                    }
+                }
+                rc.chunk[prec] = chunk;
+                rc.nblk[prec] = (int)br.size();
+                if (upload(h, &rc.d_kbeg[prec], rkb[prec], true)) return 1;
+                if (upload(h, &rc.d_kend[prec], rke[prec], true)) return 1;
+                if (upload(h, &rc.d_blk_row[prec], br, true)) return 1;
+                if (upload(h, &rc.d_blk_k0[prec], bk, true)) return 1;
+            }
+            h->rcombos.push_back(rc);
         }
     if (h->opts.verbose) {
         double q64 = 0, q32 = 0;
@@ -547,10 +675,13 @@ int build_lists(hfx_t* h) {
         fprintf(stderr, "hfx: %d shells, %zu pairs (%.0f significant), quartets fp64 %.4g fp32 %.4g\n",
                 (int)h->shells.size(), h->pairs.size(), h->npairs_sig, q64, q32);
         for (auto& c : h->combos)
-            fprintf(stderr, "hfx:   class (%d%d|%d%d) ket np %d %s R=%d: quartets %.4g / %.4g, prim quartets %.4g / %.4g\n",
+            fprintf(stderr, "hfx:   class (%d%d|%d%d) ket np %d %s: quartets %.4g / %.4g, prim quartets %.4g / %.4g\n",
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck),
-                    c.npk, c.rowmode ? "rows" : "tiles", c.R, c.nq[0], c.nq[1],
-                    c.nprimq[0], c.nprimq[1]);
+                    c.npk, c.rowmode ? "rows" : "tiles", c.nq[0], c.nq[1], c.nprimq[0], c.nprimq[1]);
+        for (auto& c : h->rcombos)
+            fprintf(stderr, "hfx:   rows (%d%d|%d%d): R=%d dlds=%d boys_lds=%d ncols=%d blocks %d / %d chunk %d / %d\n",
+                    pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck), c.R, c.dlds,
+                    c.boys_lds, c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
     }
     return 0;
 }
@@ -754,6 +885,10 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (!(h->opts.schwarz_threshold >= 0) || !(h->opts.fp32_threshold >= 0))
         return fail(h, "hfx_create: thresholds must be >= 0");
     if (h->opts.kernel_mode < 0 || h->opts.kernel_mode > 1) return fail(h, "hfx_create: kernel_mode must be 0 or 1");
+    // row-mode block shape overrides (performance tuning only; results do not depend on them)
+    if (const char* e = std::getenv("HFX_ROW_RMAX")) h->row_rmax = std::max(1, std::min(HFX_ROWS_MAX, std::atoi(e)));
+    if (const char* e = std::getenv("HFX_ROW_PAIR_KB")) h->row_pair_bytes = (size_t)std::max(1, std::atoi(e)) * 1024;
+    if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
     h->nat = nat;
     h->atno.assign(atno, atno + nat);
     h->xyz.assign(xyz, xyz + 3 * nat);
@@ -795,6 +930,38 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     auto t0 = std::chrono::steady_clock::now();
     const int n = h->nbf;
     HIPCHK(h, hfx_launch_prep(dP, h->d_scale, h->d_D, h->d_Jb, h->d_Kb, n, h->stream));
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    for (int c = 0; c < HFX_NPAIRCLASS; c++)
+        if (h->ktab[c].t.nk > 0)
+            HIPCHK(h, hfx_launch_ket_gather(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_D, n, h->stream));
+    for (const auto& c : h->rcombos) {
+        for (int prec = 0; prec < 2; prec++) {
+            if (c.nblk[prec] == 0) continue;
+            RowArgs r;
+            r.pairs = h->d_pairs;
+            r.prims = h->d_prims;
+            r.bra_list = h->d_cls_list[c.cb];
+            r.kt = h->ktab[c.ck].t;
+            r.kbeg = c.d_kbeg[prec];
+            r.kend = c.d_kend[prec];
+            r.blk_row = c.d_blk_row[prec];
+            r.blk_k0 = c.d_blk_k0[prec];
+            r.chunk = c.chunk[prec];
+            r.R = c.R;
+            r.nbra = (int)h->cls_list[c.cb].size();
+            r.ncols = c.ncols;
+            r.offd = c.offd;
+            r.bf_c = h->d_bfL[pair_class_la(c.ck)];
+            r.bf_d = h->d_bfL[pair_class_lb(c.ck)];
+            r.boys_lds = c.boys_lds;
+            r.dlds = c.dlds;
+            r.D = h->d_D;
+            r.J = h->d_Jb;
+            r.K = h->d_Kb;
+            r.n = n;
+            HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], h->stream));
+        }
+    }
     JKArgs a;
     a.pairs = h->d_pairs;
     a.prims = h->d_prims;
@@ -803,7 +970,6 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     a.K = h->d_Kb;
     a.n = n;
     double q64 = 0, q32 = 0, f64 = 0, f32 = 0;
-    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     for (const auto& c : h->combos) {
         a.bra_list = h->d_cls_list[c.cb];
         a.ket_list = c.d_ket;
@@ -814,32 +980,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
                             ncart(pair_class_la(c.ck)) * ncart(pair_class_lb(c.ck));
         for (int prec = 0; prec < 2; prec++) {
             if (c.ntiles[prec] == 0) continue;
-            if (c.rowmode) {
-                if (c.nblk[prec] == 0) continue;
-                RowArgs r;
-                r.pairs = h->d_pairs;
-                r.prims = h->d_prims;
-                r.bra_list = a.bra_list;
-                r.ket_list = a.ket_list;
-                r.kbeg = c.d_kbeg[prec];
-                r.kend = c.d_kend[prec];
-                r.blk_row = c.d_blk_row[prec];
-                r.blk_k0 = c.d_blk_k0[prec];
-                r.chunk = c.chunk[prec];
-                r.R = c.R;
-                r.nbra = a.nbra;
-                r.ncols = c.ncols;
-                r.offd = c.offd;
-                const int lc = pair_class_la(c.ck), ld = pair_class_lb(c.ck);
-                r.bf_c = h->d_bfL[lc];
-                r.bf_d = h->d_bfL[ld];
-                r.boys_lds = c.boys_lds;
-                r.D = h->d_D;
-                r.J = h->d_Jb;
-                r.K = h->d_Kb;
-                r.n = n;
-                HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], h->stream));
-            } else {
+            if (!c.rowmode) {
                 a.tile_i0 = c.d_ti0[prec];
                 a.tile_j0 = c.d_tj0[prec];
                 a.kbeg = c.d_kbeg[prec];
@@ -856,6 +997,9 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             }
         }
     }
+    for (int c = 0; c < HFX_NPAIRCLASS; c++)
+        if (h->ktab[c].t.nk > 0)
+            HIPCHK(h, hfx_launch_ket_scatter(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_Jb, n, h->stream));
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     HIPCHK(h, hfx_launch_finish(h->d_Jb, h->d_Kb, h->d_scale, dG, dJ, dK, n, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hfx_common.h"
+#include "rys_params.h"
 
 // Arguments of one J/K launch: bra pairs of class X (rows), ket pairs of class
 // Y (columns), X >= Y; both lists sorted by Schwarz bound, descending, so the
@@ -29,36 +30,80 @@ struct JKArgs {
     int n;
 };
 
-// Arguments of one row-mode J/K launch: block b owns bra rows
-// blk_row[b] .. blk_row[b] + R - 1 and the ket columns [blk_k0[b], blk_k0[b] +
-// chunk) (intersected with [kbeg, kend) of each row). The K contributions of
-// each row land in the rows of its shells A and B, accumulated in LDS over all
-// the block's kets and flushed once: LDS columns are the functions of the L_C
-// shells (then, when L_D != L_C, those of the L_D shells from column offd on);
-// bf_c / bf_d map columns back to basis functions. J_ab is reduced over the
-// block; J_cd of each ket is summed over the R rows before one flush.
+static constexpr int HFX_SUB_MAX = 8;   // primitive-count sublists of a ket class
+
+// Per-position record of a ket table: pair id, first function of C and of D,
+// and cp = cpa | cpb << 15 | (C == D) << 30 (cpa / cpb: LDS column of the
+// first function of C / of D among the functions of the L_C / L_D shells).
+struct alignas(16) KetInfo {
+    int pid, fa, fb, cp;
+};
+
+// Ket tables of one pair class: all its significant pairs, grouped into
+// sublists of equal primitive-pair count (position k in [off[s], off[s+1])
+// belongs to sublist s, Q-descending within it), stored field-major so that
+// consecutive positions (consecutive lanes) read consecutive addresses.
+struct KetTab {
+    const KetInfo* info;
+    const double* cd;     // [3][nk]: C - D
+    const double* prim;   // sublist s at prim + psoff[s]: [np_s][KETPRIM_FIELDS][nk_s]
+    double* Dcd;          // [NC * ND][nk]: density blocks D_cd, gathered per build
+    double* Jcd;          // [NC * ND][nk]: J_cd accumulators, scattered into J after the build
+    int nk;
+    int S;
+    int off[HFX_SUB_MAX + 1];
+    int np[HFX_SUB_MAX];
+    long long psoff[HFX_SUB_MAX];
+};
+
+// Arguments of one row-mode J/K launch (bra class X, ket class Y, X >= Y):
+// block b owns bra rows blk_row[b] .. blk_row[b] + R - 1 and the ket
+// positions [blk_k0[b], blk_k0[b] + chunk), intersected, sublist by sublist,
+// with each row's range [kbeg[i*S + s], kend[i*S + s]) (one precision's share
+// of the row's Schwarz prefix; for X == Y cut at the diagonal). The K
+// contributions of each row land in the rows of its shells A and B,
+// accumulated in LDS over all the block's kets and flushed once: LDS columns
+// are the functions of the L_C shells (then, when L_D != L_C, those of the
+// L_D shells from column offd on); bf_c / bf_d map columns back to basis
+// functions. With dlds the same rows of the density are staged in LDS in the
+// same layout. J_ab is reduced over the block; J_cd of each ket is summed over
+// the R rows and added to the ket table's accumulators.
 struct RowArgs {
     const ShellPair* pairs;
     const PrimPair* prims;
     const int* bra_list;
-    const int* ket_list;
+    KetTab kt;
     const int* kbeg;
     const int* kend;
     const int* blk_row;   // first of the block's R consecutive bra rows
     const int* blk_k0;
     int chunk;
-    int R;                // bra rows per block (1..HFX_ROWS_MAX): J_cd of a ket is summed over them
+    int R;                // bra rows per block (1..HFX_ROWS_MAX)
     int nbra;
     int ncols;
     int offd;
     const int* bf_c;
     const int* bf_d;
     int boys_lds;   // one- and two-root classes: stage the Boys table in LDS too
+    int dlds;       // stage the density rows of the bra shells in LDS
     const double* D;
     double* J;
     double* K;
     int n;
 };
+
+// Bytes of dynamic LDS of a row-mode launch: K rows [R][nrow][ncols] and J_ab
+// [R][nab] in double, the optional density rows [R][nrow][ncols] in the
+// arithmetic type, the optional Boys table.
+inline size_t hfx_row_lds(int nrow, int nab, int ncols, int R, int dlds, int boys, bool fp32) {
+    size_t b = (size_t)R * ((size_t)nrow * ncols + nab) * sizeof(double);
+    if (dlds) {
+        const size_t m = (size_t)R * nrow * ncols;
+        b += fp32 ? ((m + 1) / 2) * sizeof(double) : m * sizeof(double);
+    }
+    if (boys) b += (size_t)BOYS_SIZE * sizeof(double);
+    return b;
+}
 
 // Arguments of one gradient launch (same row blocks as RowArgs): every quartet's
 // sum_abcd gam_abcd d(ab|cd)/dR goes to the atoms of its four shells, summed in
@@ -97,12 +142,13 @@ struct BlockArgs {
 };
 
 static constexpr int HFX_ROWS_MAX = 4;   // bra rows per row-mode block
+static constexpr size_t HFX_ROW_LDS_MAX = 128 * 1024;   // dynamic LDS of a row-mode block
 static constexpr int HFX_TILE_ROWS = 4;   // one bra row per wavefront
 static constexpr int HFX_TILE_COLS = 64;
 
 hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, int ntiles, hipStream_t s);
-// LDS bytes of a row-mode launch for ncols columns, without the optional Boys table
-size_t hfx_row_lds_bytes(int cls_bra, int cls_ket, int ncols);
+// LDS K rows (NAt + NB) and J_ab entries (NAt * NB) per bra row of a row-mode launch
+void hfx_row_shape(int cls_bra, int cls_ket, int* nrow, int* nab);
 hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, const RowArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad(int cls_bra, int cls_ket, bool fp32, const GradArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
@@ -111,6 +157,10 @@ hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* p
                               unsigned long long* q2bits, hipStream_t s);
 hipError_t hfx_launch_prep(const double* P, const double* scale, double* D, double* Jb, double* Kb, int n,
                            hipStream_t s);
+// kt.Dcd <- D blocks of every ket of the table, kt.Jcd <- 0 (ncomp = NC * ND)
+hipError_t hfx_launch_ket_gather(const KetTab& kt, int ncomp, int nd, const double* D, int n, hipStream_t s);
+// Jb block of every ket += kt.Jcd
+hipError_t hfx_launch_ket_scatter(const KetTab& kt, int ncomp, int nd, double* Jb, int n, hipStream_t s);
 hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* scale, double* G, double* J, double* K,
                              int n, hipStream_t s);
 int hfx_nsplit(int cls_bra, int cls_ket);

@@ -116,136 +116,202 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 #endif
 
 #if HFX_PART == 2 || HFX_PART == 3
-// Row mode: one block = R bra shell pairs x a chunk of their kets (see RowArgs).
+// Row mode: one block = R bra shell pairs x a chunk of the ket table (see RowArgs).
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
     constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
+    constexpr int nrow = NAt + NB;             // LDS K rows per bra row
     extern __shared__ double smem[];
+    __shared__ int rbf_s[HFX_ROWS_MAX * nrow];  // basis function of every LDS row
     const int ncols = a.ncols;
-    const int nrow = NAt + NB;                 // LDS K rows per bra row
     const int row0 = a.blk_row[blockIdx.x];
     const int nr = min(a.R, a.nbra - row0);    // bra rows of this block
     const int nkl = a.R * nrow * ncols;
-    double* Kl = smem;                         // [R][NAt + NB][ncols]
+    double* Kl = smem;                         // [R][nrow][ncols]
     double* Jl = smem + nkl;                   // [R][NAt][NB]
-    const double* boys = hfx_dev::BOYS_TAB;
-    if constexpr (Q::NR <= 2) {
-        if (a.boys_lds) {  // Boys table staged after the accumulators
-            double* bl = smem + nkl + a.R * NAt * NB;
-            for (int k = threadIdx.x; k < BOYS_SIZE; k += blockDim.x) bl[k] = hfx_dev::BOYS_TAB[k];
-            boys = bl;
-        }
-    }
-    for (int k = threadIdx.x; k < nkl + a.R * NAt * NB; k += blockDim.x) smem[k] = 0.0;
-    __syncthreads();
+    R* Dl = reinterpret_cast<R*>(Jl + a.R * NAt * NB);   // [R][nrow][ncols] when a.dlds
+    const int ndl = a.dlds ? nkl : 0;
     const int split = blockIdx.y;
     const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
     const int sc = SPC ? split % Q::NC : 0;
     const int n = a.n;
-    // ket range of the block: union of its rows' ranges within the chunk
-    const int k0 = a.blk_k0[blockIdx.x];
-    int lo = 1 << 30, hi = 0;
-    for (int r = 0; r < nr; r++) {
-        lo = min(lo, a.kbeg[row0 + r]);
-        hi = max(hi, a.kend[row0 + r]);
-    }
-    const int j0 = max(k0, lo);
-    const int j1 = min(k0 + a.chunk, hi);
-    const bool lead = (threadIdx.x % 64) == 0;
-    // uniform trip count: every lane of a wavefront reaches the reductions
-    for (int base = j0; base < j1; base += (int)blockDim.x) {
-        const int col = base + (int)threadIdx.x;
-        const bool incol = col < j1;
-        int kp = 0;
-        ShellPair ket;
-        if (incol) {
-            kp = a.ket_list[col];
-            ket = a.pairs[kp];
+    const double* boys = hfx_dev::BOYS_TAB;
+    if constexpr (Q::NR <= 2) {
+        if (a.boys_lds) {  // Boys table staged after the other arrays
+            double* bl = reinterpret_cast<double*>(Dl) + (sizeof(R) == 8 ? ndl : (ndl + 1) / 2);
+            for (int k = threadIdx.x; k < BOYS_SIZE; k += blockDim.x) bl[k] = hfx_dev::BOYS_TAB[k];
+            boys = bl;
         }
-        R Jcd[NCt][ND];
-#pragma unroll
-        for (int c = 0; c < NCt; c++)
-#pragma unroll
-            for (int d = 0; d < ND; d++) Jcd[c][d] = R(0);
+    }
+    if (threadIdx.x < nr * nrow) {
+        const int r = threadIdx.x / nrow, i = threadIdx.x - r * nrow;
+        const ShellPair bra = a.pairs[a.bra_list[row0 + r]];
+        rbf_s[threadIdx.x] = i < NAt ? bra.fa + sa + i : bra.fb + (i - NAt);
+    }
+    for (int k = threadIdx.x; k < nkl + a.R * NAt * NB; k += blockDim.x) smem[k] = 0.0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nr * nrow * ncols && ndl > 0; k += blockDim.x) {
+        const int ri = k / ncols, col = k - ri * ncols;
+        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[col - a.offd] : a.bf_c[col];
+        Dl[k] = R(a.D[(size_t)rbf_s[ri] * n + cbf]);
+    }
+    __syncthreads();
+    const KetTab& kt = a.kt;
+    const int nk = kt.nk;
+    const int k0 = a.blk_k0[blockIdx.x], k1 = k0 + a.chunk;
+    const bool lead = (threadIdx.x % 64) == 0;
+    for (int s = 0; s < kt.S; s++) {
+        // ket range of the block in sublist s: union of its rows' ranges within the chunk
+        int lo = 1 << 30, hi = 0;
         for (int r = 0; r < nr; r++) {
-            const int row = row0 + r;
-            const int bp = a.bra_list[row];
-            const ShellPair bra = a.pairs[bp];
-            const bool act = incol && col >= a.kbeg[row] && col < a.kend[row];
-            const int fa = bra.fa + sa, fb = bra.fb;
-            JKAcc<LA, LB, LC, LD, SPA, SPC, R> acc;
-            jk_zero(acc);
-            double s = 0.0;
-            if (act) {
-                s = bra.A == bra.B ? 0.5 : 1.0;
-                if (ket.A == ket.B) s *= 0.5;
-                if (bp == kp) s *= 0.5;
-                R eri[Q::NOUT];
-                eri_quartet<LA, LB, LC, LD, SPA, SPC, R>(bra, ket, a.prims, sa, sc, dev_tables(boys), eri);
-                jk_accumulate<LA, LB, LC, LD, SPA, SPC, R>(eri, fa, fb, ket.fa + sc, ket.fb, a.D, n, acc);
-                const R rs = R(s);
+            lo = min(lo, a.kbeg[(row0 + r) * kt.S + s]);
+            hi = max(hi, a.kend[(row0 + r) * kt.S + s]);
+        }
+        const int j0 = max(k0, lo), j1 = min(k1, hi);
+        if (j0 >= j1) continue;
+        const int soff = kt.off[s], nks = kt.off[s + 1] - soff, npk = kt.np[s];
+        const double* sprim = kt.prim + kt.psoff[s];
+        // uniform trip count: every lane of a wavefront reaches the reductions
+        for (int base = j0; base < j1; base += (int)blockDim.x) {
+            const int col = base + (int)threadIdx.x;
+            const bool incol = col < j1;
+            KetInfo ki{0, 0, 0, 0};
+            double cdx = 0, cdy = 0, cdz = 0;
+            R Dcd[NCt][ND];
+            if (incol) {
+                ki = kt.info[col];
+                cdx = kt.cd[col];
+                cdy = kt.cd[nk + col];
+                cdz = kt.cd[2 * nk + col];
 #pragma unroll
                 for (int c = 0; c < NCt; c++)
 #pragma unroll
-                    for (int d = 0; d < ND; d++) Jcd[c][d] += rs * acc.Jcd[c][d];
-                const int cc = ket.cpa + sc, cd = a.offd + ket.cpb;
-                double* Kr = Kl + r * nrow * ncols;
-#pragma unroll
-                for (int i = 0; i < NAt; i++) {
-#pragma unroll
-                    for (int c = 0; c < NCt; c++) atomicAdd(&Kr[i * ncols + cc + c], s * double(acc.Kac[i][c]));
-#pragma unroll
-                    for (int d = 0; d < ND; d++) atomicAdd(&Kr[i * ncols + cd + d], s * double(acc.Kad[i][d]));
-                }
-#pragma unroll
-                for (int b = 0; b < NB; b++) {
-#pragma unroll
-                    for (int c = 0; c < NCt; c++)
-                        atomicAdd(&Kr[(NAt + b) * ncols + cc + c], s * double(acc.Kbc[b][c]));
-#pragma unroll
-                    for (int d = 0; d < ND; d++)
-                        atomicAdd(&Kr[(NAt + b) * ncols + cd + d], s * double(acc.Kbd[b][d]));
-                }
+                    for (int d = 0; d < ND; d++) Dcd[c][d] = R(kt.Dcd[(size_t)((sc + c) * ND + d) * nk + col]);
             }
-            // J_ab of row r: reduce over the wavefront, one LDS add per wavefront
-#pragma unroll
-            for (int i = 0; i < NAt; i++)
-#pragma unroll
-                for (int j = 0; j < NB; j++) {
-                    const double v = wave_sum(2.0 * s * double(acc.Jab[i][j]));
-                    if (lead && v != 0.0) atomicAdd(&Jl[(r * NAt + i) * NB + j], v);
-                }
-        }
-        if (incol) {
-            const int fc = ket.fa + sc, fd = ket.fb;
+            const int cc = (ki.cp & 0x7fff) + sc, cd = a.offd + ((ki.cp >> 15) & 0x7fff);
+            const double sket = (ki.cp >> 30) & 1 ? 0.5 : 1.0;
+            const KetPrimsSoA kp{sprim + (col - soff), npk, nks};
+            R Jcd[NCt][ND];
 #pragma unroll
             for (int c = 0; c < NCt; c++)
 #pragma unroll
-                for (int d = 0; d < ND; d++) {
-                    const double v = 2.0 * double(Jcd[c][d]);
-                    if (v != 0.0) unsafeAtomicAdd(&a.J[(size_t)(fc + c) * n + fd + d], v);
+                for (int d = 0; d < ND; d++) Jcd[c][d] = R(0);
+            for (int r = 0; r < nr; r++) {
+                const int row = row0 + r;
+                const int bp = a.bra_list[row];
+                const ShellPair bra = a.pairs[bp];
+                const bool act = incol && col >= a.kbeg[row * kt.S + s] && col < a.kend[row * kt.S + s];
+                const int fa = bra.fa + sa, fb = bra.fb;
+                JKAcc<LA, LB, LC, LD, SPA, SPC, R> acc;
+                jk_zero(acc);
+                double sf = 0.0;
+                if (act) {
+                    sf = bra.A == bra.B ? 0.5 * sket : sket;
+                    if (bp == ki.pid) sf *= 0.5;
+                    R eri[Q::NOUT];
+                    eri_quartet_k<LA, LB, LC, LD, SPA, SPC, R>(bra, a.prims, cdx, cdy, cdz, kp, sa, sc,
+                                                               dev_tables(boys), eri);
+                    JKDens<LA, LB, LC, LD, SPA, SPC, R> dn;
+#pragma unroll
+                    for (int i = 0; i < NAt; i++)
+#pragma unroll
+                        for (int j = 0; j < NB; j++) dn.ab[i][j] = R(a.D[(size_t)(fa + i) * n + fb + j]);
+#pragma unroll
+                    for (int c = 0; c < NCt; c++)
+#pragma unroll
+                        for (int d = 0; d < ND; d++) dn.cd[c][d] = Dcd[c][d];
+                    if (ndl > 0) {
+                        const R* Dr = Dl + r * nrow * ncols;
+#pragma unroll
+                        for (int i = 0; i < NAt; i++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) dn.ac[i][c] = Dr[i * ncols + cc + c];
+#pragma unroll
+                            for (int d = 0; d < ND; d++) dn.ad[i][d] = Dr[i * ncols + cd + d];
+                        }
+#pragma unroll
+                        for (int b = 0; b < NB; b++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) dn.bc[b][c] = Dr[(NAt + b) * ncols + cc + c];
+#pragma unroll
+                            for (int d = 0; d < ND; d++) dn.bd[b][d] = Dr[(NAt + b) * ncols + cd + d];
+                        }
+                    } else {
+                        const int fc = ki.fa + sc, fd = ki.fb;
+#pragma unroll
+                        for (int i = 0; i < NAt; i++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) dn.ac[i][c] = R(a.D[(size_t)(fa + i) * n + fc + c]);
+#pragma unroll
+                            for (int d = 0; d < ND; d++) dn.ad[i][d] = R(a.D[(size_t)(fa + i) * n + fd + d]);
+                        }
+#pragma unroll
+                        for (int b = 0; b < NB; b++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) dn.bc[b][c] = R(a.D[(size_t)(fb + b) * n + fc + c]);
+#pragma unroll
+                            for (int d = 0; d < ND; d++) dn.bd[b][d] = R(a.D[(size_t)(fb + b) * n + fd + d]);
+                        }
+                    }
+                    jk_contract<LA, LB, LC, LD, SPA, SPC, R>(eri, dn, acc);
+                    const R rs = R(sf);
+#pragma unroll
+                    for (int c = 0; c < NCt; c++)
+#pragma unroll
+                        for (int d = 0; d < ND; d++) Jcd[c][d] += rs * acc.Jcd[c][d];
+                    double* Kr = Kl + r * nrow * ncols;
+#pragma unroll
+                    for (int i = 0; i < NAt; i++) {
+#pragma unroll
+                        for (int c = 0; c < NCt; c++) atomicAdd(&Kr[i * ncols + cc + c], sf * double(acc.Kac[i][c]));
+#pragma unroll
+                        for (int d = 0; d < ND; d++) atomicAdd(&Kr[i * ncols + cd + d], sf * double(acc.Kad[i][d]));
+                    }
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+#pragma unroll
+                        for (int c = 0; c < NCt; c++)
+                            atomicAdd(&Kr[(NAt + b) * ncols + cc + c], sf * double(acc.Kbc[b][c]));
+#pragma unroll
+                        for (int d = 0; d < ND; d++)
+                            atomicAdd(&Kr[(NAt + b) * ncols + cd + d], sf * double(acc.Kbd[b][d]));
+                    }
                 }
+                // J_ab of row r: reduce over the wavefront, one LDS add per wavefront
+#pragma unroll
+                for (int i = 0; i < NAt; i++)
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        const double v = wave_sum(2.0 * sf * double(acc.Jab[i][j]));
+                        if (lead && v != 0.0) atomicAdd(&Jl[(r * NAt + i) * NB + j], v);
+                    }
+            }
+            if (incol) {
+#pragma unroll
+                for (int c = 0; c < NCt; c++)
+#pragma unroll
+                    for (int d = 0; d < ND; d++) {
+                        const double v = 2.0 * double(Jcd[c][d]);
+                        if (v != 0.0) unsafeAtomicAdd(&kt.Jcd[(size_t)((sc + c) * ND + d) * nk + col], v);
+                    }
+            }
         }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < nr * nrow * ncols; k += blockDim.x) {
         const double v = Kl[k];
         if (v == 0.0) continue;
-        const int r = k / (nrow * ncols), rem = k - r * nrow * ncols;
-        const int i = rem / ncols, col = rem - i * ncols;
-        const ShellPair bra = a.pairs[a.bra_list[row0 + r]];
-        const int rbf = i < NAt ? bra.fa + sa + i : bra.fb + (i - NAt);
+        const int ri = k / ncols, col = k - ri * ncols;
         const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[col - a.offd] : a.bf_c[col];
-        unsafeAtomicAdd(&a.K[(size_t)rbf * n + cbf], v);
+        unsafeAtomicAdd(&a.K[(size_t)rbf_s[ri] * n + cbf], v);
     }
     for (int k = threadIdx.x; k < nr * NAt * NB; k += blockDim.x) {
         const double v = Jl[k];
         if (v == 0.0) continue;
         const int r = k / (NAt * NB), ij = k - r * NAt * NB;
-        const ShellPair bra = a.pairs[a.bra_list[row0 + r]];
-        unsafeAtomicAdd(&a.J[(size_t)(bra.fa + sa + ij / NB) * n + bra.fb + ij % NB], v);
+        unsafeAtomicAdd(&a.J[(size_t)rbf_s[r * nrow + ij / NB] * n + rbf_s[r * nrow + NAt + ij % NB]], v);
     }
 }
 
@@ -380,6 +446,30 @@ __global__ void prep_kernel(const double* __restrict__ P, const double* __restri
     }
 }
 
+// D_cd of every ket of a table (field-major) and zeroed J_cd accumulators.
+__global__ void ket_gather_kernel(KetTab kt, int ncomp, int nd, const double* __restrict__ D, int n) {
+    const size_t tot = (size_t)ncomp * kt.nk;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
+        const int comp = (int)(idx / kt.nk), k = (int)(idx - (size_t)comp * kt.nk);
+        const KetInfo ki = kt.info[k];
+        const int c = comp / nd, d = comp - c * nd;
+        kt.Dcd[idx] = D[(size_t)(ki.fa + c) * n + ki.fb + d];
+        kt.Jcd[idx] = 0.0;
+    }
+}
+
+// Jb += J_cd of every ket (each ket is a distinct shell pair, so every element
+// has one writer).
+__global__ void ket_scatter_kernel(KetTab kt, int ncomp, int nd, double* __restrict__ Jb, int n) {
+    const size_t tot = (size_t)ncomp * kt.nk;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
+        const int comp = (int)(idx / kt.nk), k = (int)(idx - (size_t)comp * kt.nk);
+        const KetInfo ki = kt.info[k];
+        const int c = comp / nd, d = comp - c * nd;
+        Jb[(size_t)(ki.fa + c) * n + ki.fb + d] += kt.Jcd[idx];
+    }
+}
+
 __global__ void finish_kernel(const double* __restrict__ Jb, const double* __restrict__ Kb,
                               const double* __restrict__ scale, double* G, double* J, double* K, int n) {
     const size_t nn = (size_t)n * n;
@@ -429,12 +519,14 @@ typedef HFX_REAL_T RowReal;
 template <int X, int Y>
 hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
     constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
-    const size_t lds = ((size_t)a.R * (row_nab<HFX_CLS(X), HFX_CLS(Y)>() * a.ncols + NAt * ncart(pair_class_lb(X))) +
-                        (a.boys_lds ? BOYS_SIZE : 0)) * sizeof(double);
+    constexpr int NB = ncart(pair_class_lb(X));
+    const size_t lds = hfx_row_lds(NAt + NB, NAt * NB, a.ncols, a.R, a.dlds, a.boys_lds, sizeof(RowReal) == 4);
+    if (lds > HFX_ROW_LDS_MAX || a.R < 1 || a.R > HFX_ROWS_MAX || a.kt.S < 1 || a.kt.S > HFX_SUB_MAX)
+        return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void*)jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 32 * 1024);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, HFX_ROW_LDS_MAX);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
@@ -474,14 +566,15 @@ hipError_t block_xy(const BlockArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 template <int X, int Y>
-size_t row_lds_xy(int ncols) {  // without the optional Boys table
+void row_shape_xy(int* nrow, int* nab) {
     constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
-    return ((size_t)row_nab<HFX_CLS(X), HFX_CLS(Y)>() * ncols + NAt * ncart(pair_class_lb(X))) * sizeof(double);
+    *nrow = NAt + ncart(pair_class_lb(X));
+    *nab = NAt * ncart(pair_class_lb(X));
 }
 typedef hipError_t (*blk_fn)(const BlockArgs&, hipStream_t);
-typedef size_t (*lds_fn)(int);
+typedef void (*shape_fn)(int*, int*);
 const blk_fn BLK_TABLE[6][6] = HFX_TABLE(block_xy);
-const lds_fn LDS_TABLE[6][6] = HFX_TABLE(row_lds_xy);
+const shape_fn SHAPE_TABLE[6][6] = HFX_TABLE(row_shape_xy);
 #endif
 
 }  // namespace
@@ -532,9 +625,10 @@ int hfx_nsplit(int cb, int ck) {
     return nsplit(pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck));
 }
 
-size_t hfx_row_lds_bytes(int cb, int ck, int ncols) {
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return 0;
-    return LDS_TABLE[cb][ck](ncols);
+void hfx_row_shape(int cb, int ck, int* nrow, int* nab) {
+    *nrow = *nab = 0;
+    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return;
+    SHAPE_TABLE[cb][ck](nrow, nab);
 }
 
 hipError_t hfx_launch_jk_rows(int cb, int ck, bool fp32, const RowArgs& a, int nblocks, hipStream_t s) {
@@ -593,6 +687,24 @@ hipError_t hfx_launch_prep(const double* P, const double* scale, double* D, doub
     unsigned nb = (unsigned)((nn + 255) / 256);
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(prep_kernel, dim3(nb), dim3(256), 0, s, P, scale, D, Jb, Kb, n);
+    return hipGetLastError();
+}
+
+hipError_t hfx_launch_ket_gather(const KetTab& kt, int ncomp, int nd, const double* D, int n, hipStream_t s) {
+    const size_t tot = (size_t)ncomp * kt.nk;
+    if (tot == 0) return hipSuccess;
+    unsigned nb = (unsigned)((tot + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(ket_gather_kernel, dim3(nb), dim3(256), 0, s, kt, ncomp, nd, D, n);
+    return hipGetLastError();
+}
+
+hipError_t hfx_launch_ket_scatter(const KetTab& kt, int ncomp, int nd, double* Jb, int n, hipStream_t s) {
+    const size_t tot = (size_t)ncomp * kt.nk;
+    if (tot == 0) return hipSuccess;
+    unsigned nb = (unsigned)((tot + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(ket_scatter_kernel, dim3(nb), dim3(256), 0, s, kt, ncomp, nd, Jb, n);
     return hipGetLastError();
 }
 
