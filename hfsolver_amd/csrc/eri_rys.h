@@ -181,6 +181,7 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const PrimPair* __res
                                      double cdy_, double cdz_, const KP& kp, int sa, int sc, const RysTables& tab,
                                      R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT]) {
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    using PT = R;
     constexpr int NR = Q::NR;
 #pragma unroll
     for (int i = 0; i < Q::NOUT; i++) eri[i] = R(0);
@@ -210,37 +211,49 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const PrimPair* __res
     for (int k1 = 0; k1 < nkp; k1++) {
         const KetPrim Qp = kp.get(k1);
         for (int i1 = 0; i1 < bra.np; i1++) {
-            const PrimPair P = prims[bra.p0 + i1];
-            const double p = P.p, q = Qp.p, pq = p + q;
-            const double ip = P.ip, iq = Qp.ip;
-            const double isq = hfx_rsqrt(pq);
-            const double ipq = isq * isq;
-            const double rho = p * q * ipq;
-            const double PQx = P.Px - Qp.Px, PQy = P.Py - Qp.Py, PQz = P.Pz - Qp.Pz;
-            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = HFX_TWO_PI_52 * P.K * Qp.K * ip * iq * isq;
-            double u[NR], w[NR];
-            if constexpr (NR <= 2)
-                rys_roots_small<NR>(T, u, w, tab.boys);
-            else
-                rys_roots<NR>(T, u, w, tab);
+            const PrimPair P = hfx_ld_prim(prims + bra.p0 + i1);
+            // primitive-level arithmetic in the kernel's type (FP32 kernels: roots
+            // and weights from the single-precision rules for one and two roots)
+            const PT p = PT(P.p), q = PT(Qp.p), pq = p + q;
+            const PT ip = PT(P.ip), iq = PT(Qp.ip);
+            const PT isq = hfx_rsqrt(pq);
+            const PT ipq = isq * isq;
+            const PT rho = p * q * ipq;
+            const PT PQx = PT(P.Px - Qp.Px), PQy = PT(P.Py - Qp.Py), PQz = PT(P.Pz - Qp.Pz);
+            const PT T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const PT pref = PT(HFX_TWO_PI_52) * PT(P.K) * PT(Qp.K) * ip * iq * isq;
+            PT u[NR], w[NR];
+            if constexpr (NR <= 2) {
+                if constexpr (sizeof(PT) == 4)
+                    rys_roots_small_f<NR>(T, u, w, tab.boysf);
+                else
+                    rys_roots_small<NR>(T, u, w, tab.boys);
+            } else {
+                double ud[NR], wd[NR];
+                rys_roots<NR>(double(T), ud, wd, tab);
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    u[r] = PT(ud[r]);
+                    w[r] = PT(wd[r]);
+                }
+            }
 #pragma unroll
             for (int r = 0; r < NR; r++) {
-                const double ur = u[r];
-                const double t1 = ur * rho;
-                const R b00 = R(0.5 * ur * ipq);
-                const R b10 = R(0.5 * ip * (1.0 - t1 * ip));
-                const R b01 = R(0.5 * iq * (1.0 - t1 * iq));
-                const double fp = t1 * ip, fq = t1 * iq;
+                const PT ur = u[r];
+                const PT t1 = ur * rho;
+                const R b00 = R(PT(0.5) * ur * ipq);
+                const R b10 = R(PT(0.5) * ip * (PT(1) - t1 * ip));
+                const R b01 = R(PT(0.5) * iq * (PT(1) - t1 * iq));
+                const PT fp = t1 * ip, fq = t1 * iq;
                 R Ix[Q::XA][LB + 1][Q::XC][LD + 1];
                 R Iy[Q::XA][LB + 1][Q::XC][LD + 1];
                 R Iz[Q::XA][LB + 1][Q::XC][LD + 1];
-                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAx - fp * PQx), R(Qp.QCx + fq * PQx), b10, b01, b00, abx,
-                                                     cdx, R(1), sel_a[0], sel_c[0], Ix);
-                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAy - fp * PQy), R(Qp.QCy + fq * PQy), b10, b01, b00, aby,
-                                                     cdy, R(1), sel_a[1], sel_c[1], Iy);
-                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(P.PAz - fp * PQz), R(Qp.QCz + fq * PQz), b10, b01, b00, abz,
-                                                     cdz, R(w[r] * pref), sel_a[2], sel_c[2], Iz);
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(PT(P.PAx) - fp * PQx), R(PT(Qp.QCx) + fq * PQx), b10, b01, b00,
+                                                     abx, cdx, R(1), sel_a[0], sel_c[0], Ix);
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(PT(P.PAy) - fp * PQy), R(PT(Qp.QCy) + fq * PQy), b10, b01, b00,
+                                                     aby, cdy, R(1), sel_a[1], sel_c[1], Iy);
+                rys_dir<LA, LB, LC, LD, SPA, SPC, R>(R(PT(P.PAz) - fp * PQz), R(PT(Qp.QCz) + fq * PQz), b10, b01, b00,
+                                                     abz, cdz, R(w[r] * pref), sel_a[2], sel_c[2], Iz);
 #pragma unroll
                 for (int a = 0; a < Q::NAt; a++) {
                     const int ax = SPA ? 0 : cart_exp(LA, a, 0), ay = SPA ? 0 : cart_exp(LA, a, 1),

@@ -54,6 +54,10 @@ BOYS_K = 9
 BOYS_MMAX = 3                      # highest order needed (two roots: F_0..F_3)
 BOYS_NCOL = BOYS_MMAX + BOYS_K     # F_0 .. F_11 per grid point
 BOYS_NPT = int(BOYS_TMAX / BOYS_DT) + 1
+# single-precision copy on the same grid for the FP32 kernels: BOYSF_K terms,
+# error < (1/16)^5/5! ~ 8e-9 relative, below float rounding
+BOYSF_K = 5
+BOYSF_NCOL = BOYS_MMAX + BOYSF_K   # F_0 .. F_7 per grid point
 
 mp.mp.dps = DPS
 
@@ -153,7 +157,10 @@ def main():
            "#define BOYS_K %d" % BOYS_K,
            "#define BOYS_NCOL %d" % BOYS_NCOL,
            "#define BOYS_NPT %d" % BOYS_NPT,
-           "#define BOYS_SIZE %d" % (BOYS_NCOL * BOYS_NPT)]
+           "#define BOYS_SIZE %d" % (BOYS_NCOL * BOYS_NPT),
+           "#define BOYSF_K %d" % BOYSF_K,
+           "#define BOYSF_NCOL %d" % BOYSF_NCOL,
+           "#define BOYSF_SIZE %d" % (BOYSF_NCOL * BOYS_NPT)]
     with open(os.path.join(os.path.dirname(out), "rys_params.h"), "w") as f:
         f.write("\n".join(hdr) + "\n")
     lines = ["// Generated by gen_rys_tables.py - do not edit. Data of rys_params.h;",
@@ -181,9 +188,14 @@ def main():
     lines.append("};")
     # Boys table: BOYS_TAB[i * BOYS_NCOL + m] = F_m(i * BOYS_DT)
     lines.append("RYS_TABLE_QUALIFIER double BOYS_TAB[BOYS_SIZE] = {")
-    for i in range(BOYS_NPT):
-        fm = boys_moments(mp.mpf(i) * mp.mpf(BOYS_DT), BOYS_NCOL - 1)
+    fms = [boys_moments(mp.mpf(i) * mp.mpf(BOYS_DT), BOYS_NCOL - 1) for i in range(BOYS_NPT)]
+    for fm in fms:
         lines.append("  " + ", ".join("%.17e" % float(v) for v in fm) + ",")
+    lines.append("};")
+    # its single-precision copy: BOYSF_TAB[i * BOYSF_NCOL + m] = F_m(i * BOYS_DT), m < BOYSF_NCOL
+    lines.append("RYS_TABLE_QUALIFIER float BOYSF_TAB[BOYSF_SIZE] = {")
+    for fm in fms:
+        lines.append("  " + ", ".join("%.9ef" % float(v) for v in fm[:BOYSF_NCOL]) + ",")
     lines.append("};")
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -61,6 +61,68 @@ struct PrimPair {
     double pad;
 };
 
+static_assert(sizeof(ShellPair) == 80 && sizeof(PrimPair) == 80, "record layouts read field by field below");
+
+// Loads of read-only data whose address is the same for the whole wavefront:
+// on the device through the constant address space, so that the compiler can
+// use scalar loads (they stay vector loads when the address diverges).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HFX_AS4 __attribute__((address_space(4)))
+#endif
+
+HFX_HD HFX_INLINE int hfx_ld_int(const int* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const HFX_AS4 int*)p;
+#else
+    return *p;
+#endif
+}
+
+HFX_HD HFX_INLINE PrimPair hfx_ld_prim(const PrimPair* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const HFX_AS4 double* d = (const HFX_AS4 double*)p;
+    PrimPair r;
+    r.p = d[0];
+    r.ip = d[1];
+    r.Px = d[2];
+    r.Py = d[3];
+    r.Pz = d[4];
+    r.PAx = d[5];
+    r.PAy = d[6];
+    r.PAz = d[7];
+    r.K = d[8];
+    r.pad = 0.0;
+    return r;
+#else
+    return *p;
+#endif
+}
+
+HFX_HD HFX_INLINE ShellPair hfx_ld_pair(const ShellPair* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const HFX_AS4 int* q = (const HFX_AS4 int*)p;
+    const HFX_AS4 double* d = (const HFX_AS4 double*)(q + 12);
+    ShellPair r;
+    r.A = q[0];
+    r.B = q[1];
+    r.fa = q[2];
+    r.fb = q[3];
+    r.p0 = q[4];
+    r.np = q[5];
+    r.cls = q[6];
+    r.cpa = q[7];
+    r.cpb = q[8];
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    r.ABx = d[0];
+    r.ABy = d[1];
+    r.ABz = d[2];
+    r.Q = d[3];
+    return r;
+#else
+    return *p;
+#endif
+}
+
 // Fields of a ket primitive pair in a field-major ket table: p, 1/p, P, P - C, K.
 static constexpr int KETPRIM_FIELDS = 9;
 
@@ -70,5 +132,6 @@ struct PrimExp {
 };
 
 static constexpr double HFX_PI = 3.14159265358979323846;
+static constexpr double HFX_SQRT_PI = 1.77245385090551602730;
 // 2 pi^(5/2)
 static constexpr double HFX_TWO_PI_52 = 34.98683665524972;

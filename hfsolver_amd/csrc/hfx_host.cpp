@@ -106,7 +106,7 @@ struct hfx_engine {
     // row-mode J/K of one class combination over the whole ket table
     struct RowCombo {
         int cb, ck;
-        int R = 1, dlds = 0, boys_lds = 0, ncols = 0, offd = 0;
+        int R = 1, dlds = 0, ncols = 0, offd = 0;
         int nblk[2] = {0, 0};
         int chunk[2] = {0, 0};
         int* d_kbeg[2] = {nullptr, nullptr};
@@ -117,7 +117,7 @@ struct hfx_engine {
     std::vector<RowCombo> rcombos;
     // tuning of the row-mode blocks (HFX_ROW_RMAX, HFX_ROW_PAIR_KB, HFX_ROW_DLDS)
     int row_rmax = HFX_ROWS_MAX;
-    size_t row_pair_bytes = 64 * 1024;
+    size_t row_pair_bytes = 32 * 1024;
     int row_dlds = 1;
     double npairs_sig = 0;
     // device memory
@@ -189,7 +189,7 @@ void free_geometry(hfx_t* h) {
 
 RysTables host_tables() {
     return RysTables{hfx_host_tables::RYS_CHEB, hfx_host_tables::RYS_HERM_U, hfx_host_tables::RYS_HERM_W,
-                     hfx_host_tables::BOYS_TAB};
+                     hfx_host_tables::BOYS_TAB, hfx_host_tables::BOYSF_TAB};
 }
 
 void rys_roots_rt(int nr, double T, double* u, double* w) {
@@ -594,11 +594,10 @@ int build_lists(hfx_t* h) {
             rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
             int nrow = 0, nab = 0;
             hfx_row_shape(cb, ck, &nrow, &nab);
-            const int nr = (pair_class_la(cb) + pair_class_lb(cb) + lc + ld) / 2 + 1;
             int dlds = h->row_dlds;
-            if (dlds && hfx_row_lds(nrow, nab, rc.ncols, 1, 1, 0, false) > h->lds_budget) dlds = 0;
-            const size_t need = hfx_row_lds(nrow, nab, rc.ncols, 1, dlds, 0, false);
-            // rows per block: as many (up to row_rmax) as stay within row_pair_bytes (two blocks per CU)
+            if (dlds && hfx_row_lds(nrow, nab, rc.ncols, 1, 1, false) > h->lds_budget) dlds = 0;
+            const size_t need = hfx_row_lds(nrow, nab, rc.ncols, 1, dlds, false);
+            // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
             int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
             if (R == 0 && need <= h->lds_budget) R = 1;
             const bool rowmode = h->opts.kernel_mode == 0 && R >= 1 && S <= HFX_SUB_MAX && rc.ncols < 32768;
@@ -606,8 +605,6 @@ int build_lists(hfx_t* h) {
             if (!rowmode) continue;
             rc.R = R;
             rc.dlds = dlds;
-            const size_t lds_rows = hfx_row_lds(nrow, nab, rc.ncols, R, dlds, 0, false);
-            rc.boys_lds = nr <= 2 && lds_rows + BOYS_SIZE * sizeof(double) <= std::max(lds_rows, (size_t)80 * 1024);
             // blocks: (first row, first ket position) for groups of R rows, chunks of the
             // table positions that meet at least one of the group's ranges
             for (int prec = 0; prec < 2; prec++) {
@@ -679,9 +676,9 @@ int build_lists(hfx_t* h) {
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck),
                     c.npk, c.rowmode ? "rows" : "tiles", c.nq[0], c.nq[1], c.nprimq[0], c.nprimq[1]);
         for (auto& c : h->rcombos)
-            fprintf(stderr, "hfx:   rows (%d%d|%d%d): R=%d dlds=%d boys_lds=%d ncols=%d blocks %d / %d chunk %d / %d\n",
+            fprintf(stderr, "hfx:   rows (%d%d|%d%d): R=%d dlds=%d ncols=%d blocks %d / %d chunk %d / %d\n",
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck), c.R, c.dlds,
-                    c.boys_lds, c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
+                    c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
     }
     return 0;
 }
@@ -953,7 +950,6 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.offd = c.offd;
             r.bf_c = h->d_bfL[pair_class_la(c.ck)];
             r.bf_d = h->d_bfL[pair_class_lb(c.ck)];
-            r.boys_lds = c.boys_lds;
             r.dlds = c.dlds;
             r.D = h->d_D;
             r.J = h->d_Jb;

@@ -84,7 +84,6 @@ struct RowArgs {
     int offd;
     const int* bf_c;
     const int* bf_d;
-    int boys_lds;   // one- and two-root classes: stage the Boys table in LDS too
     int dlds;       // stage the density rows of the bra shells in LDS
     const double* D;
     double* J;
@@ -94,14 +93,13 @@ struct RowArgs {
 
 // Bytes of dynamic LDS of a row-mode launch: K rows [R][nrow][ncols] and J_ab
 // [R][nab] in double, the optional density rows [R][nrow][ncols] in the
-// arithmetic type, the optional Boys table.
-inline size_t hfx_row_lds(int nrow, int nab, int ncols, int R, int dlds, int boys, bool fp32) {
+// arithmetic type.
+inline size_t hfx_row_lds(int nrow, int nab, int ncols, int R, int dlds, bool fp32) {
     size_t b = (size_t)R * ((size_t)nrow * ncols + nab) * sizeof(double);
     if (dlds) {
         const size_t m = (size_t)R * nrow * ncols;
         b += fp32 ? ((m + 1) / 2) * sizeof(double) : m * sizeof(double);
     }
-    if (boys) b += (size_t)BOYS_SIZE * sizeof(double);
     return b;
 }
 

@@ -31,7 +31,7 @@ namespace hfx_dev {
 namespace {
 
 __device__ HFX_INLINE RysTables dev_tables(const double* boys = hfx_dev::BOYS_TAB) {
-    return RysTables{hfx_dev::RYS_CHEB, hfx_dev::RYS_HERM_U, hfx_dev::RYS_HERM_W, boys};
+    return RysTables{hfx_dev::RYS_CHEB, hfx_dev::RYS_HERM_U, hfx_dev::RYS_HERM_W, boys, hfx_dev::BOYSF_TAB};
 }
 
 __device__ HFX_INLINE double wave_sum(double v) {
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
     extern __shared__ double smem[];
     __shared__ int rbf_s[HFX_ROWS_MAX * nrow];  // basis function of every LDS row
     const int ncols = a.ncols;
-    const int row0 = a.blk_row[blockIdx.x];
+    const int row0 = hfx_ld_int(a.blk_row + blockIdx.x);
     const int nr = min(a.R, a.nbra - row0);    // bra rows of this block
     const int nkl = a.R * nrow * ncols;
     double* Kl = smem;                         // [R][nrow][ncols]
@@ -137,14 +137,6 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
     const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
     const int sc = SPC ? split % Q::NC : 0;
     const int n = a.n;
-    const double* boys = hfx_dev::BOYS_TAB;
-    if constexpr (Q::NR <= 2) {
-        if (a.boys_lds) {  // Boys table staged after the other arrays
-            double* bl = reinterpret_cast<double*>(Dl) + (sizeof(R) == 8 ? ndl : (ndl + 1) / 2);
-            for (int k = threadIdx.x; k < BOYS_SIZE; k += blockDim.x) bl[k] = hfx_dev::BOYS_TAB[k];
-            boys = bl;
-        }
-    }
     if (threadIdx.x < nr * nrow) {
         const int r = threadIdx.x / nrow, i = threadIdx.x - r * nrow;
         const ShellPair bra = a.pairs[a.bra_list[row0 + r]];
@@ -160,14 +152,14 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
     __syncthreads();
     const KetTab& kt = a.kt;
     const int nk = kt.nk;
-    const int k0 = a.blk_k0[blockIdx.x], k1 = k0 + a.chunk;
+    const int k0 = hfx_ld_int(a.blk_k0 + blockIdx.x), k1 = k0 + a.chunk;
     const bool lead = (threadIdx.x % 64) == 0;
     for (int s = 0; s < kt.S; s++) {
         // ket range of the block in sublist s: union of its rows' ranges within the chunk
         int lo = 1 << 30, hi = 0;
         for (int r = 0; r < nr; r++) {
-            lo = min(lo, a.kbeg[(row0 + r) * kt.S + s]);
-            hi = max(hi, a.kend[(row0 + r) * kt.S + s]);
+            lo = min(lo, hfx_ld_int(a.kbeg + (row0 + r) * kt.S + s));
+            hi = max(hi, hfx_ld_int(a.kend + (row0 + r) * kt.S + s));
         }
         const int j0 = max(k0, lo), j1 = min(k1, hi);
         if (j0 >= j1) continue;
@@ -200,9 +192,10 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
                 for (int d = 0; d < ND; d++) Jcd[c][d] = R(0);
             for (int r = 0; r < nr; r++) {
                 const int row = row0 + r;
-                const int bp = a.bra_list[row];
-                const ShellPair bra = a.pairs[bp];
-                const bool act = incol && col >= a.kbeg[row * kt.S + s] && col < a.kend[row * kt.S + s];
+                const int bp = hfx_ld_int(a.bra_list + row);
+                const ShellPair bra = hfx_ld_pair(a.pairs + bp);
+                const bool act = incol && col >= hfx_ld_int(a.kbeg + row * kt.S + s) &&
+                                 col < hfx_ld_int(a.kend + row * kt.S + s);
                 const int fa = bra.fa + sa, fb = bra.fb;
                 JKAcc<LA, LB, LC, LD, SPA, SPC, R> acc;
                 jk_zero(acc);
@@ -212,7 +205,7 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
                     if (bp == ki.pid) sf *= 0.5;
                     R eri[Q::NOUT];
                     eri_quartet_k<LA, LB, LC, LD, SPA, SPC, R>(bra, a.prims, cdx, cdy, cdz, kp, sa, sc,
-                                                               dev_tables(boys), eri);
+                                                               dev_tables(), eri);
                     JKDens<LA, LB, LC, LD, SPA, SPC, R> dn;
 #pragma unroll
                     for (int i = 0; i < NAt; i++)
@@ -520,7 +513,7 @@ template <int X, int Y>
 hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
     constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
     constexpr int NB = ncart(pair_class_lb(X));
-    const size_t lds = hfx_row_lds(NAt + NB, NAt * NB, a.ncols, a.R, a.dlds, a.boys_lds, sizeof(RowReal) == 4);
+    const size_t lds = hfx_row_lds(NAt + NB, NAt * NB, a.ncols, a.R, a.dlds, sizeof(RowReal) == 4);
     if (lds > HFX_ROW_LDS_MAX || a.R < 1 || a.R > HFX_ROWS_MAX || a.kt.S < 1 || a.kt.S > HFX_SUB_MAX)
         return hipErrorInvalidValue;
     static bool attr_set = false;

@@ -17,6 +17,7 @@ struct RysTables {
     const double* hu;    // RYS_HERM_U
     const double* hw;    // RYS_HERM_W
     const double* boys;  // BOYS_TAB (global memory, or a copy in LDS)
+    const float* boysf;  // BOYSF_TAB, single-precision Boys table of the FP32 kernels
 };
 
 HFX_HD HFX_INLINE double hfx_rsqrt(double x) {
@@ -27,11 +28,44 @@ HFX_HD HFX_INLINE double hfx_rsqrt(double x) {
 #endif
 }
 
+HFX_HD HFX_INLINE float hfx_rsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ::rsqrtf(x);
+#else
+    return 1.0f / std::sqrt(x);
+#endif
+}
+
+// 1/x in single precision: the hardware reciprocal (1 ulp) on the device.
+HFX_HD HFX_INLINE float hfx_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+// 1/x for positive normal x: hardware reciprocal and two Newton steps on the
+// device (within an ulp or two of the rounded quotient), a division on the host.
+HFX_HD HFX_INLINE double hfx_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // One- and two-root rules in closed form from the Boys moments F_0..F_{2n-1}.
 // F_m(T) for T < BOYS_TMAX: Taylor series about the nearest grid point T0,
 //   F_m(T) = sum_{k < BOYS_K} F_{m+k}(T0) (T0 - T)^k / k!;
 // beyond: F_0 = (1/2) sqrt(pi/T) (erfc(sqrt T) < 1e-16 F_0 there) and the
 // upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable for T > m.
+// For one root beyond BOYS_TMAX, e^{-T} < 2e-15 F_0, so u = F_1/F_0 = 1/(2T)
+// to that relative accuracy and no exponential is evaluated.
 // n = 1: u = F_1/F_0, w = F_0. n = 2: the monic orthogonal quadratic
 // u^2 + a u + b of the moment sequence, roots u_2 = (-a + sqrt(a^2 - 4b))/2,
 // u_1 = b / u_2, weights w_1 = (F_1 - u_2 F_0)/(u_1 - u_2), w_2 = F_0 - w_1.
@@ -54,28 +88,93 @@ HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const dou
             for (int k = BOYS_K - 2; k >= 0; k--) acc = row[m + k] + acc * xk[k + 1];
             F[m] = acc;
         }
+        if constexpr (NR == 1) {
+            u[0] = F[1] * hfx_rcp(F[0]);
+            w[0] = F[0];
+        }
     } else {
-        const double et = exp(-T);
-        const double it2 = 0.5 / T;
-        F[0] = 0.5 * sqrt(HFX_PI / T);
+        const double rt = hfx_rsqrt(T);
+        const double it2 = 0.5 * rt * rt;
+        F[0] = 0.5 * HFX_SQRT_PI * rt;
+        if constexpr (NR == 1) {
+            u[0] = it2;
+            w[0] = F[0];
+        } else {
+            const double et = exp(-T);
 #pragma unroll
-        for (int m = 0; m + 1 < NM; m++) F[m + 1] = ((2 * m + 1) * F[m] - et) * it2;
+            for (int m = 0; m + 1 < NM; m++) F[m + 1] = ((2 * m + 1) * F[m] - et) * it2;
+        }
     }
-    if (NR == 1) {
-        u[0] = F[1] / F[0];
-        w[0] = F[0];
-    } else {
-        const double m0 = F[0], m1 = F[1], m2 = F[NM > 2 ? 2 : 0], m3 = F[NM > 3 ? 3 : 0];
-        const double id = 1.0 / (m0 * m2 - m1 * m1);
+    if constexpr (NR == 2) {
+        const double m0 = F[0], m1 = F[1], m2 = F[2], m3 = F[3];
+        const double id = hfx_rcp(m0 * m2 - m1 * m1);
         const double a = (m1 * m2 - m0 * m3) * id;
         const double b = (m1 * m3 - m2 * m2) * id;
         const double u2 = 0.5 * (-a + sqrt(a * a - 4.0 * b));
-        const double u1 = b / u2;
-        const double w1 = (m1 - u2 * m0) / (u1 - u2);
+        const double u1 = b * hfx_rcp(u2);
+        const double w1 = (m1 - u2 * m0) * hfx_rcp(u1 - u2);
         u[0] = u1;
         w[0] = w1;
-        u[NR - 1] = u2;
-        w[NR - 1] = m0 - w1;
+        u[1] = u2;
+        w[1] = m0 - w1;
+    }
+}
+
+// The same rules in single precision for the FP32 kernels, from BOYSF_TAB
+// (BOYSF_K Taylor terms, relative error below float rounding); beyond
+// BOYS_TMAX the two-root rule keeps e^{-T} (float exp).
+template <int NR>
+HFX_HD HFX_INLINE void rys_roots_small_f(float T, float* u, float* w, const float* boysf) {
+    static_assert(NR == 1 || NR == 2, "closed form for one or two roots only");
+    constexpr int NM = 2 * NR;
+    float F[NM];
+    if (T < (float)BOYS_TMAX) {
+        const int i = (int)(T * (float)BOYS_INV_DT + 0.5f);
+        const float x = (float)i * (float)BOYS_DT - T;
+        float xk[BOYSF_K];
+#pragma unroll
+        for (int k = 1; k < BOYSF_K; k++) xk[k] = x * (1.0f / k);
+        const float* row = boysf + i * BOYSF_NCOL;
+#pragma unroll
+        for (int m = 0; m < NM; m++) {
+            float acc = row[m + BOYSF_K - 1];
+#pragma unroll
+            for (int k = BOYSF_K - 2; k >= 0; k--) acc = row[m + k] + acc * xk[k + 1];
+            F[m] = acc;
+        }
+        if constexpr (NR == 1) {
+            u[0] = F[1] * hfx_rcp(F[0]);
+            w[0] = F[0];
+        }
+    } else {
+        const float rt = hfx_rsqrt(T);
+        const float it2 = 0.5f * rt * rt;
+        F[0] = 0.5f * (float)HFX_SQRT_PI * rt;
+        if constexpr (NR == 1) {
+            u[0] = it2;
+            w[0] = F[0];
+        } else {
+#if defined(__HIP_DEVICE_COMPILE__)
+            const float et = __expf(-T);
+#else
+            const float et = std::exp(-T);
+#endif
+#pragma unroll
+            for (int m = 0; m + 1 < NM; m++) F[m + 1] = ((2 * m + 1) * F[m] - et) * it2;
+        }
+    }
+    if constexpr (NR == 2) {
+        const float m0 = F[0], m1 = F[1], m2 = F[2], m3 = F[3];
+        const float id = hfx_rcp(m0 * m2 - m1 * m1);
+        const float a = (m1 * m2 - m0 * m3) * id;
+        const float b = (m1 * m3 - m2 * m2) * id;
+        const float u2 = 0.5f * (-a + std::sqrt(a * a - 4.0f * b));
+        const float u1 = b * hfx_rcp(u2);
+        const float w1 = (m1 - u2 * m0) * hfx_rcp(u1 - u2);
+        u[0] = u1;
+        w[0] = w1;
+        u[1] = u2;
+        w[1] = m0 - w1;
     }
 }
 

@@ -14,3 +14,6 @@
 #define BOYS_NCOL 12
 #define BOYS_NPT 289
 #define BOYS_SIZE 3468
+#define BOYSF_K 5
+#define BOYSF_NCOL 8
+#define BOYSF_SIZE 2312

@@ -16,7 +16,8 @@ namespace chk_tables {
 namespace {
 
 RysTables tabs() {
-    return RysTables{chk_tables::RYS_CHEB, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W, chk_tables::BOYS_TAB};
+    return RysTables{chk_tables::RYS_CHEB, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W, chk_tables::BOYS_TAB,
+                     chk_tables::BOYSF_TAB};
 }
 
 void make_pair(int la, int lb, const double* A, const double* B, int na, const double* aa, const double* ca, int nb,
@@ -167,6 +168,19 @@ double chk_rys(int nr, double T, double* u, double* w) {
     switch (nr) {
         case -1: rys_roots_small<1>(T, u, w, chk_tables::BOYS_TAB); break;
         case -2: rys_roots_small<2>(T, u, w, chk_tables::BOYS_TAB); break;
+        case -11:
+        case -12: {  // single-precision rules of the FP32 kernels
+            float uf[2], wf[2];
+            if (nr == -11)
+                rys_roots_small_f<1>((float)T, uf, wf, chk_tables::BOYSF_TAB);
+            else
+                rys_roots_small_f<2>((float)T, uf, wf, chk_tables::BOYSF_TAB);
+            for (int i = 0; i < -nr - 10; i++) {
+                u[i] = uf[i];
+                w[i] = wf[i];
+            }
+            break;
+        }
         case 1: rys_roots<1>(T, u, w, tabs()); break;
         case 2: rys_roots<2>(T, u, w, tabs()); break;
         case 3: rys_roots<3>(T, u, w, tabs()); break;
