@@ -275,11 +275,40 @@ int build_shells(hfx_t* h, int nbf, const int* nprim, const int* istart, const d
 }
 
 // --------------------------------------------------------------- pairs ---
+// Primitive pairs are screened with the primitive Schwarz estimate of an s-type
+// product, q = |K| sqrt(2 pi^{5/2} / (p^2 sqrt(2p))) = sqrt((ab|ab)) for s
+// functions: a pair with q * q_max below 1e-8 times the Schwarz threshold (q_max
+// over all primitive pairs) is left out. Every shell pair keeps its place in the
+// pair list (index i (i + 1) / 2 + j for shells i >= j).
+static constexpr double HFX_PRIM_SCREEN = 1e-8;
+
+double prim_bound(double K, double p) {
+    return std::fabs(K) * std::sqrt(2.0 * HFX_PI * HFX_PI * std::sqrt(HFX_PI) / (p * p * std::sqrt(2.0 * p)));
+}
+
 void build_pairs(hfx_t* h) {
     h->pairs.clear();
     h->prims.clear();
     h->pexp.clear();
     const int ns = (int)h->shells.size();
+    // largest primitive bound of the molecule
+    double qmax = 0;
+    for (int i = 0; i < ns; i++)
+        for (int j = 0; j <= i; j++) {
+            const Shell& sa = h->shells[i];
+            const Shell& sb = h->shells[j];
+            const double* ra = &h->xyz[3 * sa.atom];
+            const double* rb = &h->xyz[3 * sb.atom];
+            const double r2 = (ra[0] - rb[0]) * (ra[0] - rb[0]) + (ra[1] - rb[1]) * (ra[1] - rb[1]) +
+                              (ra[2] - rb[2]) * (ra[2] - rb[2]);
+            for (int pa = 0; pa < sa.nprim; pa++)
+                for (int pb = 0; pb < sb.nprim; pb++) {
+                    const double a = h->alpha[sa.prim0 + pa], b = h->alpha[sb.prim0 + pb], p = a + b;
+                    const double K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-a * b / p * r2);
+                    qmax = std::max(qmax, prim_bound(K, p));
+                }
+        }
+    const double qcut = qmax > 0 ? HFX_PRIM_SCREEN * h->opts.schwarz_threshold / qmax : 0.0;
     for (int i = 0; i < ns; i++)
         for (int j = 0; j <= i; j++) {
             int A = i, B = j;
@@ -318,10 +347,11 @@ void build_pairs(hfx_t* h) {
                     pp.PAy = pp.Py - ra[1];
                     pp.PAz = pp.Pz - ra[2];
                     pp.K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-a * b / p * r2);
+                    if (prim_bound(pp.K, p) < qcut) continue;
                     h->prims.push_back(pp);
                     h->pexp.push_back(PrimExp{a, b});
                 }
-            sp.np = (int)h->prims.size() - sp.p0;
+            sp.np = (int)h->prims.size() - sp.p0;   // 0: the pair drops out through Q = 0
             h->pairs.push_back(sp);
         }
 }
@@ -390,6 +420,15 @@ int block_chunk(double tot, int ngroups) {
     return std::max(256, (int)std::ceil(tot / target_blocks / 256.0) * 256);
 }
 
+// Primitive-pair count of a ket sublist: counts are rounded up to these bucket
+// sizes (beyond the last, to a multiple of it) so that a class has few sublists.
+int np_bucket(int np) {
+    static const int B[] = {1, 3, 6, 9, 14, 18, 27, 36, 54, 72, 100, 144, 196, 256};
+    for (int b : B)
+        if (np <= b) return b;
+    return (np + 255) / 256 * 256;
+}
+
 // Field-major ket table of pair class c from its sublists (see KetTab).
 int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) {
     hfx_engine::KetTable& T = h->ktab[c];
@@ -404,7 +443,7 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
     long long ps = 0;
     for (int s = 0; s < S; s++) {
         t.off[s] = nk;
-        t.np[s] = h->pairs[subl[s][0]].np;
+        t.np[s] = np_bucket(h->pairs[subl[s][0]].np);
         t.psoff[s] = ps;
         nk += (int)subl[s].size();
         ps += (long long)t.np[s] * KETPRIM_FIELDS * (long long)subl[s].size();
@@ -423,9 +462,10 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
             cd[k] = sp.ABx;
             cd[(size_t)nk + k] = sp.ABy;
             cd[2 * (size_t)nk + k] = sp.ABz;
-            for (int i = 0; i < t.np[s]; i++) {
-                const PrimPair& q = h->prims[sp.p0 + i];
-                const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, q.K};
+            for (int i = 0; i < t.np[s]; i++) {   // padding: last primitive pair with K = 0
+                const PrimPair& q = h->prims[sp.p0 + std::min(i, sp.np - 1)];
+                const double K = i < sp.np ? q.K : 0.0;
+                const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, K};
                 for (int fi = 0; fi < KETPRIM_FIELDS; fi++)
                     prim[(size_t)t.psoff[s] + ((size_t)i * KETPRIM_FIELDS + fi) * nks + j] = f[fi];
             }
@@ -473,13 +513,13 @@ int build_lists(hfx_t* h) {
     for (int c = 0; c < HFX_NPAIRCLASS; c++) {
         if (h->cls_list[c].empty()) continue;
         std::vector<int> nps;
-        for (int pid : h->cls_list[c]) nps.push_back(h->pairs[pid].np);
+        for (int pid : h->cls_list[c]) nps.push_back(np_bucket(h->pairs[pid].np));
         std::sort(nps.begin(), nps.end());
         nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
         for (int npk : nps) {
             std::vector<int> kl;
             for (int pid : h->cls_list[c])
-                if (h->pairs[pid].np == npk) kl.push_back(pid);
+                if (np_bucket(h->pairs[pid].np) == npk) kl.push_back(pid);
             subl[c].push_back(kl);
         }
         if (build_ket_table(h, c, subl[c])) return 1;
@@ -499,14 +539,15 @@ int build_lists(hfx_t* h) {
             size_t first_combo = h->combos.size();
             for (int s = 0; s < S; s++) {
                 const std::vector<int>& kl = subl[ck][s];
-                const int npk = h->pairs[kl[0]].np;
+                const int npk = np_bucket(h->pairs[kl[0]].np);
                 const int soff = S <= HFX_SUB_MAX ? h->ktab[ck].t.off[s] : 0;
                 const int nk = (int)kl.size();
-                std::vector<double> kq(nk);
+                std::vector<double> kq(nk), cnp(nk + 1, 0.0);   // cnp: prefix sums of primitive-pair counts
                 std::vector<int> kpos(nk);
                 for (int j = 0; j < nk; j++) {
                     kq[j] = h->pairs[kl[j]].Q;
                     kpos[j] = pos[kl[j]];
+                    cnp[j + 1] = cnp[j] + h->pairs[kl[j]].np;
                 }
                 // number of kets with Q_i Q_j >= t: prefix of the descending list
                 auto prefix = [&](double qi, double t) -> int {
@@ -538,7 +579,7 @@ int build_lists(hfx_t* h) {
                         ke[i] = prec == 0 ? k64[i] : kmax[i];
                         const double nqi = std::max(0, ke[i] - kb[i]);
                         cmb.nq[prec] += nqi;
-                        cmb.nprimq[prec] += nqi * h->pairs[bl[i]].np * npk;
+                        if (ke[i] > kb[i]) cmb.nprimq[prec] += h->pairs[bl[i]].np * (cnp[ke[i]] - cnp[kb[i]]);
                         rkb[prec][(size_t)i * S + s] = soff + kb[i];
                         rke[prec][(size_t)i * S + s] = soff + std::max(kb[i], ke[i]);
                     }

@@ -30,7 +30,7 @@ struct JKArgs {
     int n;
 };
 
-static constexpr int HFX_SUB_MAX = 8;   // primitive-count sublists of a ket class
+static constexpr int HFX_SUB_MAX = 16;  // primitive-count sublists of a ket class
 
 // Per-position record of a ket table: pair id, first function of C and of D,
 // and cp = cpa | cpb << 15 | (C == D) << 30 (cpa / cpb: LDS column of the
@@ -40,9 +40,11 @@ struct alignas(16) KetInfo {
 };
 
 // Ket tables of one pair class: all its significant pairs, grouped into
-// sublists of equal primitive-pair count (position k in [off[s], off[s+1])
-// belongs to sublist s, Q-descending within it), stored field-major so that
-// consecutive positions (consecutive lanes) read consecutive addresses.
+// sublists by primitive-pair count rounded up to a bucket size np[s] (position
+// k in [off[s], off[s+1]) belongs to sublist s, Q-descending within it; a ket
+// with fewer primitive pairs is padded with copies of its last one with K = 0),
+// stored field-major so that consecutive positions (consecutive lanes) read
+// consecutive addresses.
 struct KetTab {
     const KetInfo* info;
     const double* cd;     // [3][nk]: C - D
