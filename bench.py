@@ -2,13 +2,16 @@
 """J+K Fock builds/s on a 64-water cluster in 6-31G** (n = 1600), one MI355X.
 
 One step = one direct two-electron Fock build G(P) = J - K/2 (Schwarz
-threshold 1e-12) with the density resident in HBM, through the C ABI's
-hfx_fock_2e_device. The headline value is the mixed-precision form (quartets
-with Q_ab Q_cd < 1e-5 in FP32 arithmetic, everything else FP64); the all-FP64
-form is timed beside it, and the mixed form's |dE| = |1/2 tr P (G_mixed - G_fp64)|
-and max |dF_ij| = max |G_mixed - G_fp64| are reported.
+threshold 1e-12) at the converged FP64 SCF density, resident in HBM, through
+the C ABI's hfx_fock_2e_device. The headline value is the mixed-precision form
+(quartets with Q_ab Q_cd < 1e-5 in FP32 arithmetic, everything else FP64); the
+all-FP64 form is timed beside it. Reported beside: |E_tot(mixed SCF) -
+E_tot(FP64 SCF)|, the first-order |dE| = |1/2 tr P (G_mixed - G_fp64)| and
+max |dF_ij| at the density, the rate with host buffers (PCIe included), the
+analytic-gradient times, and the oracle timed on a sample of the same
+quartets (cpu_baseline).
 
-    python bench.py [--steps K] [--warmup W] [--nwater 64] [--density core|scf]
+    python bench.py [--steps K] [--warmup W] [--nwater 64] [--density scf|core]
 """
 import argparse
 import json
@@ -90,13 +93,15 @@ def cpu_baseline(eng, basis, nq_total, target_s=12.0, seed=0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nwater", type=int, default=64)
     ap.add_argument("--seed", type=int, default=64)
-    ap.add_argument("--density", choices=["core", "scf"], default="core")
+    ap.add_argument("--density", choices=["core", "scf"], default="scf",
+                    help="timing density: converged FP64 SCF density (default) or the core-Hamiltonian guess")
     ap.add_argument("--fp32-threshold", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the analytic-gradient timing")
     ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known bytes) to scale HBM counters")
@@ -122,6 +127,7 @@ def main():
     from hfsolver_amd.basis import Basis
     from hfsolver_amd.clusters import water_cluster
     from hfsolver_amd.engine import Engine
+    from hfsolver_amd.scf import doscf, nuclear_energy
 
     atno, xyz = water_cluster(args.nwater, seed=args.seed)
     b = Basis(atno, xyz)
@@ -138,13 +144,24 @@ def main():
     H = T + V
     log("  one-electron %.2f s" % (time.time() - t))
     nocc = int(sum(atno)) // 2
-    lam, C = scipy.linalg.eigh(H, S, lower=True, driver="gvd")
-    P = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+    Enuc = nuclear_energy(atno, xyz)
+    scf = {}
+    r64 = None
     if args.density == "scf":
-        from hfsolver_amd.scf import doscf
-        r = doscf(H, S, nocc, lambda X: e64.fock_2e(X)[0], Nscf=60, tolE=1e-8, tolP=1e-6, diis=True)
-        P = r["P"]
-        log("  density: SCF converged in %d iterations" % r["iterations"])
+        # FP64 and mixed-precision SCF from the same start (P = 0), DIIS
+        kw = dict(Nscf=80, tolE=1e-9, tolP=1e-6, diis=True)
+        for name, eng in (("fp64", e64), ("mixed", emx)):
+            t = time.time()
+            r = doscf(H, S, nocc, lambda X, e=eng: e.fock_2e(X)[0], **kw)
+            scf[name] = {"Etot": r["E0"] + Enuc, "iterations": r["iterations"], "seconds": time.time() - t}
+            log("  %s SCF: E_tot %.10f Eh in %d iterations (%.1f s)" % (name, scf[name]["Etot"], r["iterations"],
+                                                                     time.time() - t))
+            if name == "fp64":
+                r64 = r
+        P = r64["P"]
+    else:
+        lam, C = scipy.linalg.eigh(H, S, lower=True, driver="gvd")
+        P = 2.0 * C[:, :nocc] @ C[:, :nocc].T
     dev = torch.device("cuda:0")
     dP = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
     dG64 = torch.zeros(n * n, dtype=torch.float64, device=dev)
@@ -152,26 +169,56 @@ def main():
     torch.cuda.synchronize()
 
     def run(eng, dG, steps):
-        sts = []
+        # fock_2e_device returns after the stream has finished: per-step wall times
+        sts, ts = [], []
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
         for _ in range(steps):
+            t0 = time.perf_counter()
             sts.append(eng.fock_2e_device(dP, dG))
+            ts.append(time.perf_counter() - t0)
         torch.cuda.synchronize()
-        return time.perf_counter() - t0, sts
+        return float(np.sum(ts)), sts, ts
 
     for _ in range(args.warmup):
         run(e64, dG64, 1)
         run(emx, dGmx, 1)
-    t64, st64 = run(e64, dG64, args.steps)
-    log("  FP64: %.4f s / build" % (t64 / args.steps))
-    tmx, stmx = run(emx, dGmx, args.steps)
-    log("  mixed: %.4f s / build" % (tmx / args.steps))
+    t64, st64, ts64 = run(e64, dG64, args.steps)
+    log("  FP64: %.4f s / build (median %.4f)" % (t64 / args.steps, np.median(ts64)))
+    tmx, stmx, tsmx = run(emx, dGmx, args.steps)
+    log("  mixed: %.4f s / build (median %.4f)" % (tmx / args.steps, np.median(tsmx)))
     G64 = dG64.cpu().numpy().reshape(n, n)
     Gmx = dGmx.cpu().numpy().reshape(n, n)
     dE = 0.5 * float(np.sum(P * (Gmx - G64)))
     dF = float(np.abs(Gmx - G64).max())
     E64 = 0.5 * float(np.sum(P * (2 * H + G64)))
+    # host buffers: P upload and G download included (PCIe)
+    nh = min(args.steps, 3)
+    emx.fock_2e(P)
+    t0 = time.perf_counter()
+    for _ in range(nh):
+        emx.fock_2e(P)
+    t_host = (time.perf_counter() - t0) / nh
+    log("  mixed, host buffers: %.4f s / build" % t_host)
+
+    grad = None
+    if not args.no_gradient and r64 is not None:
+        from hfsolver_amd.grad import energy_weighted_density
+        W = energy_weighted_density(r64["C"], r64["lam"], nocc)
+        Pc = 2.0 * r64["C"][:, :nocc] @ r64["C"][:, :nocc].T
+        e64.grad_2e(Pc)
+        t0 = time.perf_counter()
+        e64.grad_2e(Pc)
+        t2 = time.perf_counter() - t0
+        emx.grad_2e(Pc)
+        t0 = time.perf_counter()
+        emx.grad_2e(Pc)
+        t2mx = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        e64.grad_1e(Pc, W)
+        t1 = time.perf_counter() - t0
+        log("  gradient: two-electron %.3f s (FP64), %.3f s (mixed); one-electron (host) %.3f s" % (t2, t2mx, t1))
+        grad = {"grad_2e_seconds_fp64": t2, "grad_2e_seconds_mixed": t2mx, "grad_1e_seconds_host": t1,
+                "gradients_per_s_mixed": 1.0 / (t2mx + t1)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(e64, b, st64[0].quartets_fp64)
 
@@ -184,8 +231,9 @@ def main():
     frac_mx = (fmx64 / PEAK_FP64 + fmx32 / PEAK_FP32) / jkmx
     value = args.steps / tmx
     traffic = None
-    if args.nwater == 64 and args.seed == 64 and args.density == "core":
+    if args.nwater == 64 and args.seed == 64:
         traffic = measured_traffic()
+    nint = stmx[0].integrals_fp64 + stmx[0].integrals_fp32
     out = {
         "metric": "J+K Fock builds/s, 64 H2O 6-31G**: mixed-FP32 vs FP64 GPU, |dE|<1e-6 Eh",
         "value": value,
@@ -198,15 +246,23 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+f32",
-        "data": "synthetic seeded water cluster (seed %d), %s density" % (args.seed, args.density),
-        "config": {"workload": "direct J+K Fock build, %d waters 6-31G** (n=%d), Schwarz 1e-12, FP32 below 1e-5"
-                               % (args.nwater, n),
+        "data": "synthetic seeded water cluster (seed %d), %s density" %
+                (args.seed, "converged FP64 SCF" if args.density == "scf" else "core-Hamiltonian"),
+        "config": {"workload": "direct J+K Fock build, %d waters 6-31G** (n=%d), Schwarz 1e-12, FP32 below %g"
+                               % (args.nwater, n, args.fp32_threshold),
                    "nbf": n, "quartets_fp64": stmx[0].quartets_fp64, "quartets_fp32": stmx[0].quartets_fp32},
+        "median_ms_mixed": 1e3 * float(np.median(tsmx)),
         "fp64_builds_per_s": args.steps / t64,
+        "median_ms_fp64": 1e3 * float(np.median(ts64)),
         "mixed_vs_fp64_speedup": t64 / tmx,
-        "mixed_abs_dE": abs(dE),
+        "mixed_builds_per_s_host_buffers": 1.0 / t_host,
+        "contracted_integrals_per_s_mixed": nint * args.steps / tmx,
+        "scf_abs_dE_mixed_vs_fp64": abs(scf["mixed"]["Etot"] - scf["fp64"]["Etot"]) if scf else None,
+        "scf": scf or None,
+        "mixed_abs_dE_at_density": abs(dE),
         "mixed_max_dF": dF,
         "E_elec_fp64_at_density": E64,
+        "gradient": grad,
         "roofline": {"bound": "fp64-valu", "achieved": roof64 / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": roof64 / PEAK_FP64, "traffic": traffic,
                      "kernel": "jk_row_kernel/jk_kernel (all class launches of one FP64 build)",

@@ -1006,7 +1006,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     a.J = h->d_Jb;
     a.K = h->d_Kb;
     a.n = n;
-    double q64 = 0, q32 = 0, f64 = 0, f32 = 0;
+    double q64 = 0, q32 = 0, f64 = 0, f32 = 0, i64 = 0, i32 = 0;
     for (const auto& c : h->combos) {
         a.bra_list = h->d_cls_list[c.cb];
         a.ket_list = c.d_ket;
@@ -1028,9 +1028,11 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             if (prec == 0) {
                 q64 += c.nq[0];
                 f64 += fl;
+                i64 += c.nq[0] * nint;
             } else {
                 q32 += c.nq[1];
                 f32 += fl;
+                i32 += c.nq[1] * nint;
             }
         }
     }
@@ -1049,6 +1051,8 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
         st->quartets_fp32 = q32;
         st->flops_fp64 = f64;
         st->flops_fp32 = f32;
+        st->integrals_fp64 = i64;
+        st->integrals_fp32 = i32;
     }
     return 0;
 }

@@ -27,7 +27,8 @@ class HfxOpts(ctypes.Structure):
 class HfxStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("quartets_fp64", ctypes.c_double),
                 ("quartets_fp32", ctypes.c_double), ("flops_fp64", ctypes.c_double),
-                ("flops_fp32", ctypes.c_double), ("jk_seconds", ctypes.c_double)]
+                ("flops_fp32", ctypes.c_double), ("jk_seconds", ctypes.c_double),
+                ("integrals_fp64", ctypes.c_double), ("integrals_fp32", ctypes.c_double)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/hfx.h

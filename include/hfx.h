@@ -52,6 +52,8 @@ typedef struct {
     double flops_fp64;        /* algorithmic FLOPs of the FP64 part (frozen per-class model) */
     double flops_fp32;        /* algorithmic FLOPs of the FP32 part */
     double jk_seconds;        /* J/K kernel time, HIP events on the engine's stream */
+    double integrals_fp64;    /* contracted Cartesian integrals (ab|cd) of the FP64 quartets */
+    double integrals_fp32;    /* same for the FP32 quartets */
 } hfx_stats;
 
 /* Fill *o with the defaults. Returns 0. */
