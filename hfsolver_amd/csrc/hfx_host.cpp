@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -1250,6 +1251,8 @@ int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
         ba.qidx = d_q + s;
         ba.nq = e - s;
         ba.out = d_out;
+        ba.packed = 0;
+        ba.scale = nullptr;
         HIPCHK(h, hfx_launch_block(cb, ck, ba, h->stream));
         s = e;
     }
@@ -1289,6 +1292,65 @@ int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
                         dst[((a * nB + b) * nC + c) * nD + d] = sc * src[((x1 * n2 + x2) * n3 + x3) * n4 + x4];
                     }
     }
+    return 0;
+}
+
+int hfx_int2(hfx_t* h, double* int2, long long size) {
+    if (!h || !int2) return h ? fail(h, "hfx_int2: null output") : 1;
+    const long long n = h->nbf, m = n * (n + 1) / 2, need = m * (m + 1) / 2;
+    if (size != need) return fail(h, "hfx_int2: size %lld, expected m(m+1)/2 = %lld for n = %lld", size, need, n);
+    HIPCHK(h, hipSetDevice(h->device));
+    // every unordered pair of shell pairs once (bra index >= ket index), oriented
+    // class(bra) >= class(ket); pairs without primitive pairs contribute zeros
+    const int np = (int)h->pairs.size();
+    std::vector<int> bra, ket;
+    for (int p1 = 0; p1 < np; p1++) {
+        if (h->pairs[p1].np == 0) continue;
+        for (int p2 = 0; p2 <= p1; p2++) {
+            if (h->pairs[p2].np == 0) continue;
+            const bool sw = h->pairs[p1].cls < h->pairs[p2].cls;
+            bra.push_back(sw ? p2 : p1);
+            ket.push_back(sw ? p1 : p2);
+        }
+    }
+    const long long nq = (long long)bra.size();
+    if (nq > INT32_MAX) return fail(h, "hfx_int2: %lld shell quartets exceed the explicit-quartet launcher", nq);
+    // quartets grouped by class combination
+    std::vector<int> order(nq), start(64 + 1, 0);
+    for (long long q = 0; q < nq; q++) start[h->pairs[bra[q]].cls * 8 + h->pairs[ket[q]].cls + 1]++;
+    for (int c = 0; c < 64; c++) start[c + 1] += start[c];
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (long long q = 0; q < nq; q++) order[fill[h->pairs[bra[q]].cls * 8 + h->pairs[ket[q]].cls]++] = (int)q;
+    }
+    int *d_bra = nullptr, *d_ket = nullptr, *d_q = nullptr;
+    double* d_out = nullptr;
+    if (upload(h, &d_bra, bra, false) || upload(h, &d_ket, ket, false) || upload(h, &d_q, order, false)) return 1;
+    if (dmalloc(h, (void**)&d_out, (size_t)size * sizeof(double), false)) return 1;
+    HIPCHK(h, hipMemsetAsync(d_out, 0, (size_t)size * sizeof(double), h->stream));
+    for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
+        for (int ck = 0; ck <= cb; ck++) {
+            const int s0 = start[cb * 8 + ck], s1 = start[cb * 8 + ck + 1];
+            if (s1 <= s0) continue;
+            BlockArgs ba;
+            ba.pairs = h->d_pairs;
+            ba.prims = h->d_prims;
+            ba.bra = d_bra;
+            ba.ket = d_ket;
+            ba.off = nullptr;
+            ba.qidx = d_q + s0;
+            ba.nq = s1 - s0;
+            ba.out = d_out;
+            ba.packed = 1;
+            ba.scale = h->d_scale;
+            HIPCHK(h, hfx_launch_block(cb, ck, ba, h->stream));
+        }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(int2, d_out, (size_t)size * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(d_bra);
+    (void)hipFree(d_ket);
+    (void)hipFree(d_q);
+    (void)hipFree(d_out);
     return 0;
 }
 

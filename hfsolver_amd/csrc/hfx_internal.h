@@ -139,6 +139,10 @@ struct BlockArgs {
     const int* qidx;   // quartet ids of this class combination
     int nq;
     double* out;
+    // packed mode (off unused): every integral, times the scales of its four
+    // functions, goes to out[index(i, j, k, l)] of the 8-fold packed array
+    int packed;
+    const double* scale;
 };
 
 static constexpr int HFX_ROWS_MAX = 4;   // bra rows per row-mode block

@@ -400,6 +400,14 @@ __global__ __launch_bounds__(256) void schwarz_kernel(const ShellPair* pairs, co
     if (m > 0.0) atomicMax(&q2bits[pid], (unsigned long long)__double_as_longlong(m));
 }
 
+// Position of (ij|kl) in the 8-fold packed array: ij = i(i+1)/2 + j for
+// i >= j, the same for kl, then the larger pair index first (64-bit).
+__device__ HFX_INLINE long long packed_index(long long i, long long j, long long k, long long l) {
+    const long long ij = i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+    const long long kl = k >= l ? k * (k + 1) / 2 + l : l * (l + 1) / 2 + k;
+    return ij >= kl ? ij * (ij + 1) / 2 + kl : kl * (kl + 1) / 2 + ij;
+}
+
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void block_kernel(BlockArgs a) {
     constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
@@ -414,6 +422,22 @@ __global__ __launch_bounds__(64) void block_kernel(BlockArgs a) {
     const ShellPair ket = a.pairs[a.ket[q]];
     double eri[Q::NOUT];
     eri_quartet<LA, LB, LC, LD, SPA, SPC, double>(bra, ket, a.prims, sa, sc, dev_tables(), eri);
+    if (a.packed) {
+#pragma unroll
+        for (int aa = 0; aa < Q::NAt; aa++)
+#pragma unroll
+            for (int b = 0; b < Q::NB; b++)
+#pragma unroll
+                for (int c = 0; c < Q::NCt; c++)
+#pragma unroll
+                    for (int d = 0; d < Q::ND; d++) {
+                        const int i = bra.fa + (SPA ? sa : aa), j = bra.fb + b;
+                        const int k2 = ket.fa + (SPC ? sc : c), l = ket.fb + d;
+                        const double v = eri[((aa * Q::NB + b) * Q::NCt + c) * Q::ND + d];
+                        a.out[packed_index(i, j, k2, l)] = v * a.scale[i] * a.scale[j] * a.scale[k2] * a.scale[l];
+                    }
+        return;
+    }
     double* o = a.out + a.off[q];
 #pragma unroll
     for (int aa = 0; aa < Q::NAt; aa++)

@@ -46,6 +46,7 @@ SIGNATURES = {
     "hfx_grad_1e": (_I, [_P, _P, _P, _P]),
     "hfx_set_geometry": (_I, [_P, _P]),
     "hfx_eri_shells": (_I, [_P, _I, _P, _P]),
+    "hfx_int2": (_I, [_P, _P, ctypes.c_longlong]),
     "hfx_shells": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
     "hfx_screening": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "hfx_pair_bounds": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
@@ -178,6 +179,14 @@ class Engine:
             res.append(out[o:o + s].reshape(nc(Ls[a]), nc(Ls[b]), nc(Ls[c]), nc(Ls[d])))
             o += s
         return res
+
+    def int2(self):
+        """Every unique (ij|kl) in the reference's 8-fold packed order (store mode)."""
+        n = self.basis.nbf
+        m = n * (n + 1) // 2
+        out = np.zeros(m * (m + 1) // 2)
+        self._check(lib().hfx_int2(self._h, _ptr(out), out.size))
+        return out
 
     def shells(self):
         ns = ctypes.c_int()

@@ -105,8 +105,12 @@ def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=F
 
 
 def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=False, P0=None, fp32_threshold=0.0,
-              schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0):
-    """Closed-shell RHF of a neutral molecule (src/drivers.f90:12-51)."""
+              schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0, int2=False):
+    """Closed-shell RHF of a neutral molecule (src/drivers.f90:12-51).
+
+    With int2=True the result also carries `int2`, every unique (ij|kl) in the
+    packed order the reference's rhf_gauss returns (store mode, all primitive
+    pairs kept), for consumers of the stored integrals."""
     basis = Basis(atno, xyz) if engine is None else engine.basis
     eng = engine if engine is not None else Engine(basis, schwarz_threshold=schwarz_threshold,
                                                    fp32_threshold=fp32_threshold, device=device)
@@ -127,6 +131,13 @@ def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=Fals
     out.Ekin = float(np.sum(out.P * T))
     out.Etot = out.E0 + out.Enuc
     out.basis = basis
+    out.int2 = None
+    if int2:
+        store = Engine(basis, schwarz_threshold=0.0, device=device)
+        try:
+            out.int2 = store.int2()
+        finally:
+            store.close()
     if engine is None and not keep_engine:
         eng.close()
         out.engine = None

@@ -123,6 +123,21 @@ int hfx_set_geometry(hfx_t *h, const double *xyz);
  */
 int hfx_eri_shells(hfx_t *h, int nq, const int *shells, double *out);
 
+/*
+ * Store mode: every unique two-electron integral over the basis functions,
+ * written into the 8-fold packed array int2[size], size = m(m+1)/2 with
+ * m = nbf(nbf+1)/2, at the 0-based index of src/cints.c:327-347
+ * (ij = i(i+1)/2 + j for i >= j, the same for kl, then the larger pair index
+ * first; 64-bit here).
+ * Replaces getints2 (src/gaussians_hfsolver.f90:13-34) -> getInts
+ * (src/qc.c:95-132, src/qc.h:35-39), which fill the same array on the CPU.
+ * Evaluated on the GPU in FP64; no Schwarz screening (shell pairs whose every
+ * primitive pair was screened out at hfx_create give zeros: create the handle
+ * with schwarz_threshold = 0 to keep every primitive pair). The array is
+ * computed in device memory (size * 8 bytes) and copied to int2.
+ */
+int hfx_int2(hfx_t *h, double *int2, long long size);
+
 /* Shell structure: number of shells; per shell its first basis function,
  * angular momentum and atom (arrays of length nshell, may be NULL). */
 int hfx_shells(const hfx_t *h, int *nshell, int *first_bf, int *L, int *atom);

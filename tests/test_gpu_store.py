@@ -1,0 +1,79 @@
+"""Store mode (hfx_int2): every unique (ij|kl) in the reference's packed order,
+against the compiled reference's int2 entries (tests/golden, sampled by the
+reference run) and against the oracle's full getInts restatement; and the
+reference's own molecule test program (SCF, MBPT2/3, Green's-function poles
+against known answers, src/tests/test_gaussians_molecules.f90) run with the
+GPU `gaussians` plug-in in place of src/gaussians_hfsolver.f90."""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from hfsolver_amd.basis import Basis
+from hfsolver_amd.clusters import water_cluster
+from hfsolver_amd.engine import Engine
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+MOLTEST = os.path.join(ROOT, "oracle", "_ref", "test_molecules_gpu")
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.mark.parametrize("name", ["h2o_config0", "ch4", "water2_seed2"])
+def test_int2_matches_reference_entries(golden, name):
+    g = golden(name)
+    say("int2 vs reference entries,", name)
+    eng = Engine(Basis(g["atno"], g["xyz"]), schwarz_threshold=0.0)
+    int2 = eng.int2()
+    eng.close()
+    assert len(g["int2_val"]) > 50
+    idx = [O.intindex(*(int(x) - 1 for x in q)) for q in g["int2_idx"]]   # fixture indices are 1-based
+    assert np.abs(int2[idx] - g["int2_val"]).max() < 1e-13
+
+
+def test_int2_matches_oracle_everywhere():
+    atno, xyz = water_cluster(2, seed=5)
+    say("int2 vs oracle getInts, water dimer")
+    b = Basis(atno, xyz)
+    eng = Engine(b, schwarz_threshold=0.0)
+    int2 = eng.int2()
+    eng.close()
+    ref = O.get_ints(O.Basis(atno, xyz))
+    assert int2.shape == ref.shape
+    assert np.abs(int2 - ref).max() < 1e-13
+    # with the default Schwarz threshold, primitive screening changes nothing visible
+    eng = Engine(b)
+    d = np.abs(eng.int2() - ref).max()
+    eng.close()
+    assert d < 1e-12
+
+
+def test_int2_size_is_checked():
+    eng = Engine(Basis(*water_cluster(1, seed=0)))
+    from hfsolver_amd.engine import HfxError, lib, _ptr
+    bad = np.zeros(10)
+    with pytest.raises(HfxError, match="hfx_int2: size"):
+        eng._check(lib().hfx_int2(eng._h, _ptr(bad), bad.size))
+    eng.close()
+
+
+@pytest.mark.skipif(not os.path.exists(MOLTEST), reason="reference molecule test not built (oracle/Makefile.ref)")
+def test_reference_molecule_tests_with_gpu_getints2(tmp_path):
+    say("reference test_gaussians_molecules with the GPU getints2")
+    shutil.copy(os.path.join(ROOT, "hfsolver_amd", "data", "p631ss.txt"), tmp_path / "p631ss.txt")
+    # the reference keeps n^4 work arrays on the stack (src/mbpt.f90:25-26)
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited && exec %s" % MOLTEST], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=600)
+    tail = "\n".join(r.stdout.splitlines()[-12:])
+    say(tail)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "E2    (MBPT2)" in r.stdout
