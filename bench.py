@@ -148,11 +148,11 @@ def main():
     scf = {}
     r64 = None
     if args.density == "scf":
-        # FP64 and mixed-precision SCF from the same start (P = 0), DIIS
+        # FP64 and mixed-precision SCF from the same start (P = 0), DIIS, linear algebra on the GPU
         kw = dict(Nscf=80, tolE=1e-9, tolP=1e-6, diis=True)
         for name, eng in (("fp64", e64), ("mixed", emx)):
             t = time.time()
-            r = doscf(H, S, nocc, lambda X, e=eng: e.fock_2e(X)[0], **kw)
+            r = doscf(H, S, nocc, lambda X, e=eng: e.fock_2e(X)[0], eig="device", engine=eng, **kw)
             scf[name] = {"Etot": r["E0"] + Enuc, "iterations": r["iterations"], "seconds": time.time() - t}
             log("  %s SCF: E_tot %.10f Eh in %d iterations (%.1f s)" % (name, scf[name]["Etot"], r["iterations"],
                                                                      time.time() - t))

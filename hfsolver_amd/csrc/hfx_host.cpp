@@ -66,6 +66,7 @@ struct hfx_engine {
     int nat = 0;
     std::vector<int> atno;
     std::vector<double> xyz;          // 3*nat
+    double ztot = 0;                  // sum of the nuclear charges
     std::vector<Shell> shells;
     std::vector<double> alpha, coef;  // shell-level primitives
     std::vector<double> scale;        // per basis function
@@ -737,6 +738,10 @@ int setup_geometry(hfx_t* h) {
 }
 
 // ------------------------------------------------------ one-electron ----
+// Primitive pairs whose bound on every one-electron contribution is below
+// this are skipped (far below the accuracy of the two-electron screening).
+static constexpr double HFX_ONE_E_SCREEN = 1e-17;
+
 // 1-D overlap table S[i][j] (i <= imax, j <= jmax) of a primitive pair, without
 // the exp / pi factors: S(0,0) = 1, Obara-Saika recurrences.
 void os_overlap_1d(double PA, double PB, double oo2p, int imax, int jmax, double S[8][8]) {
@@ -798,11 +803,21 @@ void one_e_block(const hfx_t* h, int A, int B, bool deriv, OneEBlock& out, const
     std::memset(out.dT, 0, sizeof out.dT);
     std::memset(out.dV, 0, sizeof out.dV);
     const int ea = la + (deriv ? 1 : 0);  // highest power needed on A
+    // bound on the primitive pair's contributions (overlap, kinetic and all
+    // nuclei, with their derivatives): |K| (pi/p)^{3/2} or |K| 2 pi/p sum Z,
+    // times (1 + a + b)^2 for the kinetic / derivative factors and
+    // (1 + |A-B|)^(la + lb + 1) for the Cartesian polynomials
+    const double poly = std::pow(1.0 + std::sqrt(r2), la + lb + 1);
     for (int pa = 0; pa < sa.nprim; pa++)
         for (int pb = 0; pb < sb.nprim; pb++) {
             const double a = h->alpha[sa.prim0 + pa], b = h->alpha[sb.prim0 + pb];
             const double p = a + b, oo2p = 0.5 / p;
-            const double K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-a * b / p * r2);
+            const double mu_r2 = a * b / p * r2;
+            if (mu_r2 > 745.0) continue;   // exp underflows to 0
+            const double K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-mu_r2);
+            const double bound = std::fabs(K) * std::max(std::pow(HFX_PI / p, 1.5), 2.0 * HFX_PI / p * h->ztot) *
+                                 (1.0 + a + b) * (1.0 + a + b) * poly;
+            if (bound < HFX_ONE_E_SCREEN) continue;
             double P[3], PA[3], PB[3];
             for (int d = 0; d < 3; d++) {
                 P[d] = (a * ra[d] + b * rb[d]) / p;
@@ -930,6 +945,8 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
     h->nat = nat;
     h->atno.assign(atno, atno + nat);
+    h->ztot = 0;
+    for (int a = 0; a < nat; a++) h->ztot += std::fabs((double)atno[a]);
     h->xyz.assign(xyz, xyz + 3 * nat);
     if (build_shells(h, nbf, nprim, istart, x, y, z, l, m, n, n2, coef, alpha)) return 1;
     int ndev = 0;

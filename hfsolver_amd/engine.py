@@ -61,6 +61,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HfxError("native library %s is not built (python -m hfsolver_amd.build)" % LIB_PATH)
+        # torch for ROCm ships its own HIP and HSA runtimes (same sonames as
+        # /opt/rocm's). Loaded first, they also serve libhfx.so, so that the
+        # process has one runtime and torch device buffers can be handed to the
+        # engine; loaded after libhfx.so they would start a second HSA runtime,
+        # which finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -15,8 +15,9 @@ coordinates, Nscf, mixing alpha, tolE, tolP) and the same semantics:
     input density, P is the last output density; Enuc = sum Z_i Z_j / R_ij,
     Ekin = sum P T, Etot = E0 + Enuc.
 Additions with no counterpart in the reference: optional DIIS extrapolation
-of F (Pulay, error FPS - SPF), a warm-start density P0, and the FP32
-threshold of the mixed-precision engine.
+of F (Pulay, error FPS - SPF), a warm-start density P0, the FP32 threshold of
+the mixed-precision engine, and eig="device" (the iteration's linear algebra
+on the GPU, the density kept in HBM).
 """
 import math
 
@@ -67,9 +68,81 @@ def _diis_extrapolate(Fs, errs):
     return sum(ci * Fi for ci, Fi in zip(c, Fs))
 
 
+def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device):
+    """doscf with the linear algebra on the GPU (torch on ROCm: rocBLAS / rocSOLVER
+    in FP64) and the density kept in HBM (engine.fock_2e_device). F C = S C lam is
+    solved as (L^-1 F L^-T) Y = Y lam, C = L^-T Y, with S = L L^T (Cholesky) -- the
+    reduction dsygvd performs for the lower triangle (src/linalg.f90:241-280)."""
+    import torch
+    dev = torch.device("cuda", device)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    n = H.shape[0]
+    Ht, St = t(H), t(S)
+    L = torch.linalg.cholesky(St)
+    Li = torch.linalg.solve_triangular(L, torch.eye(n, dtype=torch.float64, device=dev), upper=False)
+    P = torch.zeros((n, n), dtype=torch.float64, device=dev) if P0 is None else t(P0)
+    G = torch.empty((n, n), dtype=torch.float64, device=dev)
+    Escf, Fs, errs = [], [], []
+    for it in range(1, Nscf + 1):
+        engine.fock_2e_device(P, G)
+        F = Ht + G
+        E0 = 0.5 * float(torch.sum(P * (Ht + F)))
+        Escf.append(E0)
+        if callback is not None:
+            callback(it, E0, P, G)
+        Fd = F
+        if diis and bool(torch.any(P != 0)):
+            err = F @ P @ St - St @ P @ F
+            Fs.append(F)
+            errs.append(err)
+            if len(Fs) > diis_size:
+                Fs.pop(0)
+                errs.pop(0)
+            if len(Fs) >= 2:
+                m = len(Fs)
+                B = -np.ones((m + 1, m + 1))
+                B[m, m] = 0.0
+                for i in range(m):
+                    for j in range(m):
+                        B[i, j] = float(torch.sum(errs[i] * errs[j]))
+                rhs = np.zeros(m + 1)
+                rhs[m] = -1.0
+                try:
+                    c = np.linalg.solve(B, rhs)[:m]
+                    Fd = sum(float(ci) * Fi for ci, Fi in zip(c, Fs))
+                except np.linalg.LinAlgError:
+                    pass
+        A = Li @ Fd @ Li.T
+        lam, Y = torch.linalg.eigh(0.5 * (A + A.T))
+        C = Li.T @ Y
+        Pold = P
+        Co = C[:, :nocc]
+        P = 2.0 * Co @ Co.T
+        dP = math.sqrt(float(torch.sum((P - Pold) ** 2)) / 4)
+        if it > 3:
+            w = Escf[it - 4:it]
+            if dP < tolP and max(w) - min(w) < tolE:
+                cpu = lambda x: x.cpu().numpy()
+                return dict(C=cpu(C), P=cpu(P), Pin=cpu(Pold), lam=cpu(lam), E0=E0, F=cpu(F), G=cpu(G),
+                            iterations=it, dP=dP, E_iter=Escf)
+        if not diis:
+            P = (1 - alpha) * Pold + alpha * P
+    raise RuntimeError("SCF did not converge in %d iterations." % Nscf)
+
+
 def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=False, diis_size=8, P0=None,
-          callback=None):
-    """The SCF loop of src/scf.f90:72-136 with G(P) from `fock_2e`."""
+          callback=None, eig="host", engine=None, device=0):
+    """The SCF loop of src/scf.f90:72-136 with G(P) from `fock_2e`.
+
+    eig="host": LAPACK dsygvd on the lower triangle as the reference does;
+    eig="device": the same steps on the GPU (needs `engine`, whose
+    fock_2e_device is used instead of fock_2e)."""
+    if eig == "device":
+        if engine is None:
+            raise ValueError("doscf(eig='device') needs the engine")
+        return _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device)
+    if eig != "host":
+        raise ValueError("eig must be 'host' or 'device'")
     n = H.shape[0]
     P = np.zeros((n, n)) if P0 is None else np.array(P0, dtype=float)
     Escf, Fs, errs = [], [], []
@@ -105,7 +178,7 @@ def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=F
 
 
 def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=False, P0=None, fp32_threshold=0.0,
-              schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0, int2=False):
+              schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0, int2=False, eig="host"):
     """Closed-shell RHF of a neutral molecule (src/drivers.f90:12-51).
 
     With int2=True the result also carries `int2`, every unique (ij|kl) in the
@@ -118,7 +191,8 @@ def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=Fals
         S, T, V = eng.one_electron()
         H = T + V
         nelec = int(np.sum(atno))
-        r = doscf(H, S, nelec // 2, lambda P: eng.fock_2e(P)[0], Nscf, alpha, tolE, tolP, diis=diis, P0=P0)
+        r = doscf(H, S, nelec // 2, lambda P: eng.fock_2e(P)[0], Nscf, alpha, tolE, tolP, diis=diis, P0=P0,
+                  eig=eig, engine=eng, device=device)
     except Exception:
         if engine is None and not keep_engine:
             eng.close()

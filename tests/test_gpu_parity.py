@@ -153,3 +153,17 @@ def test_sixteen_waters_invariants():
             ref = O.eri(ob, f0[A] + a, f0[B] + bb, f0[C] + c, f0[D] + d)
             assert abs(blk[a, bb, c, d] - ref) < 1e-14 + 1e-13 * abs(ref)
     eng.close()
+
+
+def test_scf_device_linear_algebra_matches_host():
+    """doscf(eig='device') (rocSOLVER eigensolver, density in HBM) against the
+    host LAPACK route on the water dimer, DIIS and linear mixing."""
+    from hfsolver_amd.clusters import water_cluster
+    atno, xyz = water_cluster(2, seed=2)
+    for kw in (dict(diis=True, Nscf=80), dict(alpha=0.5, Nscf=100)):
+        say("SCF device vs host linear algebra, dimer", kw)
+        rh = rhf_gauss(atno, xyz, tolE=1e-12, tolP=1e-9, **kw)
+        rd = rhf_gauss(atno, xyz, tolE=1e-12, tolP=1e-9, eig="device", **kw)
+        assert abs(rd.Etot - rh.Etot) < 1e-10
+        assert np.abs(rd.lam - rh.lam).max() < 1e-8
+        assert np.abs(rd.P - rh.P).max() < 1e-6
