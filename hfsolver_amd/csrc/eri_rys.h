@@ -138,10 +138,13 @@ HFX_HD HFX_INLINE void rys_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R 
 }
 
 // Primitive-pair data of a ket as the quartet loop reads it: exponent sum q,
-// 1/q, centre Q, Q - C, and the prefactor K_cd.
-struct KetPrim {
-    double p, ip, Px, Py, Pz, QCx, QCy, QCz, K;
+// 1/q, centre Q, Q - C, and the prefactor K_cd (double, or float in the
+// single-precision ket tables of the FP32 kernels).
+template <typename T>
+struct KetPrimT {
+    T p, ip, Px, Py, Pz, QCx, QCy, QCz, K;
 };
+using KetPrim = KetPrimT<double>;
 
 // Ket primitives stored as PrimPair records (the ket PrimPair keeps Q - C in
 // its PA fields).
@@ -158,15 +161,16 @@ struct KetPrimsAoS {
 // Ket primitives of a ket list stored field-major, so that consecutive kets
 // (consecutive lanes) read consecutive addresses: field f of primitive k of
 // ket j at base[(k * KETPRIM_FIELDS + f) * stride + j], base already offset by j.
+template <typename T>
 struct KetPrimsSoA {
-    const double* base;
+    const T* base;
     int np;
     int stride;
     HFX_HD HFX_INLINE int count() const { return np; }
-    HFX_HD HFX_INLINE KetPrim get(int k) const {
-        const double* b = base + (size_t)k * KETPRIM_FIELDS * stride;
-        return KetPrim{b[0], b[stride], b[2 * stride], b[3 * stride], b[4 * stride],
-                       b[5 * stride], b[6 * stride], b[7 * stride], b[8 * stride]};
+    HFX_HD HFX_INLINE KetPrimT<T> get(int k) const {
+        const T* b = base + (size_t)k * KETPRIM_FIELDS * stride;
+        return KetPrimT<T>{b[0], b[stride], b[2 * stride], b[3 * stride], b[4 * stride],
+                           b[5 * stride], b[6 * stride], b[7 * stride], b[8 * stride]};
     }
 };
 
@@ -209,11 +213,11 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const PrimPair* __res
     const R cdx = R(cdx_), cdy = R(cdy_), cdz = R(cdz_);
     const int nkp = kp.count();
     for (int k1 = 0; k1 < nkp; k1++) {
-        const KetPrim Qp = kp.get(k1);
+        const auto Qp = kp.get(k1);
         for (int i1 = 0; i1 < bra.np; i1++) {
             const PrimPair P = hfx_ld_prim(prims + bra.p0 + i1);
             // primitive-level arithmetic in the kernel's type (FP32 kernels: roots
-            // and weights from the single-precision rules for one and two roots)
+            // and weights from the single-precision rules and tables)
             const PT p = PT(P.p), q = PT(Qp.p), pq = p + q;
             const PT ip = PT(P.ip), iq = PT(Qp.ip);
             const PT isq = hfx_rsqrt(pq);
@@ -229,13 +233,10 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const PrimPair* __res
                 else
                     rys_roots_small<NR>(T, u, w, tab.boys);
             } else {
-                double ud[NR], wd[NR];
-                rys_roots<NR>(double(T), ud, wd, tab);
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    u[r] = PT(ud[r]);
-                    w[r] = PT(wd[r]);
-                }
+                if constexpr (sizeof(PT) == 4)
+                    rys_roots_f<NR>(T, u, w, tab);
+                else
+                    rys_roots<NR>(T, u, w, tab);
             }
 #pragma unroll
             for (int r = 0; r < NR; r++) {

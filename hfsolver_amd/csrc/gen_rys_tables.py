@@ -44,6 +44,8 @@ H = 1.0           # interval width in T
 NINT = 60         # intervals on [0, TMAX); exp(-T) T^(m-1/2)/Gamma(m+1/2) < 1e-16 at T = 60 for m <= 9
 TMAX = H * NINT
 DEG = 13          # Chebyshev degree per interval
+DEGF = 6          # single-precision copy: the series truncated after degree 6
+                  # (dropped coefficients sum to < 1.1e-8 of |c_0| on every interval)
 DPS = 60
 # Boys-function Taylor table for the one- and two-root rules (closed form from
 # F_0..F_3): F_m(T) = sum_{k<BOYS_K} F_{m+k}(T0) (T0-T)^k / k!, T0 on a grid of
@@ -151,6 +153,8 @@ def main():
            "#define RYS_H %.17g" % H,
            "#define RYS_TMAX %.17g" % TMAX,
            "#define RYS_CHEB_SIZE %d" % o,
+           "#define RYS_DEGF %d" % DEGF,
+           "#define RYS_CHEBF_SIZE %d" % (o // (DEG + 1) * (DEGF + 1)),
            "#define BOYS_DT %.17g" % BOYS_DT,
            "#define BOYS_INV_DT %.17g" % (1 / BOYS_DT),
            "#define BOYS_TMAX %.17g" % BOYS_TMAX,
@@ -171,6 +175,14 @@ def main():
         for it in range(NINT):
             for row in table[(n, it)]:
                 lines.append("  " + ", ".join("%.17e" % v for v in row) + ",")
+    lines.append("};")
+    # single-precision truncation for the FP32 kernels, same layout with RYS_DEGF + 1 coefficients
+    lines.append("RYS_TABLE_QUALIFIER float RYS_CHEBF[RYS_CHEBF_SIZE] = {")
+    for n in range(1, NMAX + 1):
+        lines.append("// n = %d" % n)
+        for it in range(NINT):
+            for row in table[(n, it)][:DEGF + 1]:
+                lines.append("  " + ", ".join("%.9ef" % float(v) for v in row) + ",")
     lines.append("};")
     # asymptotic Gauss-Hermite data: RYS_HERM_U[n-1][i], RYS_HERM_W[n-1][i]
     hu, hw = [], []

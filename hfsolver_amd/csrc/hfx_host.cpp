@@ -191,7 +191,7 @@ void free_geometry(hfx_t* h) {
 
 RysTables host_tables() {
     return RysTables{hfx_host_tables::RYS_CHEB, hfx_host_tables::RYS_HERM_U, hfx_host_tables::RYS_HERM_W,
-                     hfx_host_tables::BOYS_TAB, hfx_host_tables::BOYSF_TAB};
+                     hfx_host_tables::BOYS_TAB, hfx_host_tables::BOYSF_TAB, hfx_host_tables::RYS_CHEBF};
 }
 
 void rys_roots_rt(int nr, double T, double* u, double* w) {
@@ -473,16 +473,20 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
             }
         }
     }
+    std::vector<float> primf(prim.begin(), prim.end());
     KetInfo* d_info = nullptr;
     double *d_cd = nullptr, *d_prim = nullptr;
+    float* d_primf = nullptr;
     if (upload(h, &d_info, info, true)) return 1;
     if (upload(h, &d_cd, cd, true)) return 1;
     if (upload(h, &d_prim, prim, true)) return 1;
+    if (upload(h, &d_primf, primf, true)) return 1;
     if (dmalloc(h, (void**)&t.Dcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
     if (dmalloc(h, (void**)&t.Jcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
     t.info = d_info;
     t.cd = d_cd;
     t.prim = d_prim;
+    t.primf = d_primf;
     return 0;
 }
 

@@ -49,6 +49,7 @@ struct KetTab {
     const KetInfo* info;
     const double* cd;     // [3][nk]: C - D
     const double* prim;   // sublist s at prim + psoff[s]: [np_s][KETPRIM_FIELDS][nk_s]
+    const float* primf;   // the same in single precision (FP32 kernels)
     double* Dcd;          // [NC * ND][nk]: density blocks D_cd, gathered per build
     double* Jcd;          // [NC * ND][nk]: J_cd accumulators, scattered into J after the build
     int nk;

@@ -31,7 +31,8 @@ namespace hfx_dev {
 namespace {
 
 __device__ HFX_INLINE RysTables dev_tables(const double* boys = hfx_dev::BOYS_TAB) {
-    return RysTables{hfx_dev::RYS_CHEB, hfx_dev::RYS_HERM_U, hfx_dev::RYS_HERM_W, boys, hfx_dev::BOYSF_TAB};
+    return RysTables{hfx_dev::RYS_CHEB, hfx_dev::RYS_HERM_U, hfx_dev::RYS_HERM_W, boys, hfx_dev::BOYSF_TAB,
+                     hfx_dev::RYS_CHEBF};
 }
 
 __device__ HFX_INLINE double wave_sum(double v) {
@@ -164,7 +165,12 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
         const int j0 = max(k0, lo), j1 = min(k1, hi);
         if (j0 >= j1) continue;
         const int soff = kt.off[s], nks = kt.off[s + 1] - soff, npk = kt.np[s];
-        const double* sprim = kt.prim + kt.psoff[s];
+        // ket primitive planes in the arithmetic type of the kernel
+        const R* sprim;
+        if constexpr (sizeof(R) == 4)
+            sprim = kt.primf + kt.psoff[s];
+        else
+            sprim = kt.prim + kt.psoff[s];
         // uniform trip count: every lane of a wavefront reaches the reductions
         for (int base = j0; base < j1; base += (int)blockDim.x) {
             const int col = base + (int)threadIdx.x;
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
             }
             const int cc = (ki.cp & 0x7fff) + sc, cd = a.offd + ((ki.cp >> 15) & 0x7fff);
             const double sket = (ki.cp >> 30) & 1 ? 0.5 : 1.0;
-            const KetPrimsSoA kp{sprim + (col - soff), npk, nks};
+            const KetPrimsSoA<R> kp{sprim + (col - soff), npk, nks};
             R Jcd[NCt][ND];
 #pragma unroll
             for (int c = 0; c < NCt; c++)

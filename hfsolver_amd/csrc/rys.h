@@ -18,6 +18,7 @@ struct RysTables {
     const double* hw;    // RYS_HERM_W
     const double* boys;  // BOYS_TAB (global memory, or a copy in LDS)
     const float* boysf;  // BOYSF_TAB, single-precision Boys table of the FP32 kernels
+    const float* chebf;  // RYS_CHEBF, single-precision truncated Chebyshev series of the FP32 kernels
 };
 
 HFX_HD HFX_INLINE double hfx_rsqrt(double x) {
@@ -214,6 +215,56 @@ HFX_HD HFX_INLINE void rys_roots(double T, double* u, double* w, const RysTables
 #pragma unroll
         for (int f = 0; f < 2 * NR; f++) {
             const double t = c[k * 2 * NR + f] + x2 * b1[f] - b2[f];
+            b2[f] = b1[f];
+            b1[f] = t;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        u[i] = c[2 * i] + x * b1[2 * i] - b2[2 * i];
+        w[i] = c[2 * i + 1] + x * b1[2 * i + 1] - b2[2 * i + 1];
+    }
+}
+
+// Offset (in floats) of the n-root block of RYS_CHEBF.
+HFX_HD constexpr int rys_offset_f(int n) {
+    int o = 0;
+    for (int m = 1; m < n; m++) o += RYS_NINT * (RYS_DEGF + 1) * 2 * m;
+    return o;
+}
+
+// Single-precision nodes and weights for the FP32 kernels: the same piecewise
+// Chebyshev series truncated after degree RYS_DEGF (relative truncation error
+// below 1.1e-8), float Clenshaw; Gauss-Hermite beyond RYS_TMAX.
+template <int NR>
+HFX_HD HFX_INLINE void rys_roots_f(float T, float* u, float* w, const RysTables& tab) {
+    static_assert(NR >= 1 && NR <= RYS_NMAX, "root count outside the tables");
+    if (T >= (float)RYS_TMAX) {
+        const float it = hfx_rcp(T);
+        const float st = hfx_rsqrt(T);
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            u[i] = (float)tab.hu[(NR - 1) * RYS_NMAX + i] * it;
+            w[i] = (float)tab.hw[(NR - 1) * RYS_NMAX + i] * st;
+        }
+        return;
+    }
+    int itv = (int)(T * (float)(1.0 / RYS_H));
+    if (itv > RYS_NINT - 1) itv = RYS_NINT - 1;
+    const float x = 2.0f * (T - (float)itv * (float)RYS_H) * (float)(1.0 / RYS_H) - 1.0f;
+    const float x2 = 2.0f * x;
+    const float* c = tab.chebf + rys_offset_f(NR) + itv * (RYS_DEGF + 1) * 2 * NR;
+    float b1[2 * NR], b2[2 * NR];
+#pragma unroll
+    for (int f = 0; f < 2 * NR; f++) {
+        b1[f] = 0.0f;
+        b2[f] = 0.0f;
+    }
+#pragma unroll
+    for (int k = RYS_DEGF; k >= 1; k--) {
+#pragma unroll
+        for (int f = 0; f < 2 * NR; f++) {
+            const float t = c[k * 2 * NR + f] + x2 * b1[f] - b2[f];
             b2[f] = b1[f];
             b1[f] = t;
         }

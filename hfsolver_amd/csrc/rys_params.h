@@ -7,6 +7,8 @@
 #define RYS_H 1
 #define RYS_TMAX 60
 #define RYS_CHEB_SIZE 25200
+#define RYS_DEGF 6
+#define RYS_CHEBF_SIZE 12600
 #define BOYS_DT 0.125
 #define BOYS_INV_DT 8
 #define BOYS_TMAX 36

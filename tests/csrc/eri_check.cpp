@@ -17,7 +17,7 @@ namespace {
 
 RysTables tabs() {
     return RysTables{chk_tables::RYS_CHEB, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W, chk_tables::BOYS_TAB,
-                     chk_tables::BOYSF_TAB};
+                     chk_tables::BOYSF_TAB, chk_tables::RYS_CHEBF};
 }
 
 void make_pair(int la, int lb, const double* A, const double* B, int na, const double* aa, const double* ca, int nb,
@@ -168,6 +168,26 @@ double chk_rys(int nr, double T, double* u, double* w) {
     switch (nr) {
         case -1: rys_roots_small<1>(T, u, w, chk_tables::BOYS_TAB); break;
         case -2: rys_roots_small<2>(T, u, w, chk_tables::BOYS_TAB); break;
+        case -21:
+        case -22:
+        case -23:
+        case -24:
+        case -25: {  // single-precision Chebyshev rules of the FP32 kernels
+            float uf[5], wf[5];
+            const float Tf = (float)T;
+            switch (nr) {
+                case -21: rys_roots_f<1>(Tf, uf, wf, tabs()); break;
+                case -22: rys_roots_f<2>(Tf, uf, wf, tabs()); break;
+                case -23: rys_roots_f<3>(Tf, uf, wf, tabs()); break;
+                case -24: rys_roots_f<4>(Tf, uf, wf, tabs()); break;
+                default: rys_roots_f<5>(Tf, uf, wf, tabs()); break;
+            }
+            for (int i = 0; i < -nr - 20; i++) {
+                u[i] = uf[i];
+                w[i] = wf[i];
+            }
+            break;
+        }
         case -11:
         case -12: {  // single-precision rules of the FP32 kernels
             float uf[2], wf[2];

@@ -86,6 +86,13 @@ def test_rys_rule_reproduces_boys_moments(lib):
             for k in range(2 * nr):
                 assert abs(np.sum(w[:nr] * u[:nr] ** k) - F[k]) <= 1e-15 * F[0] + 4e-15 * F[k]
             assert np.all(np.diff(u[:nr]) > 0) and np.all(u[:nr] > 0) and np.all(u[:nr] < 1)
+            # single-precision Chebyshev form (FP32 kernels), at T rounded to float
+            Tf = float(np.float32(T))
+            Ff = O.boys(2 * nr - 1, Tf)
+            lib.chk_rys(-20 - nr, ctypes.c_double(T), p(u), p(w))
+            assert np.all(u[:nr] > 0) and np.all(u[:nr] < 1)
+            for k in range(2 * nr):
+                assert abs(np.sum(w[:nr] * u[:nr] ** k) - Ff[k]) <= 1e-6 * Ff[0] + 4e-6 * Ff[k], (nr, T, k)
             if nr <= 2:   # closed-form rules from the Boys Taylor table
                 lib.chk_rys(-nr, ctypes.c_double(T), p(u), p(w))
                 for k in range(2 * nr):
