@@ -172,6 +172,14 @@ struct KetPrimsSoA {
         return KetPrimT<T>{b[0], b[stride], b[2 * stride], b[3 * stride], b[4 * stride],
                            b[5 * stride], b[6 * stride], b[7 * stride], b[8 * stride]};
     }
+    // exponent of the C primitive of primitive pair k
+    HFX_HD HFX_INLINE T cexp(int k) const { return base[((size_t)k * KETPRIM_FIELDS + 9) * stride]; }
+};
+
+// Ket primitives as PrimPair records plus their exponents (derivative integrals).
+struct KetPrimsAoSG : KetPrimsAoS {
+    const PrimExp* pe;
+    HFX_HD HFX_INLINE double cexp(int k) const { return pe[k].aa; }
 };
 
 // Contracted integrals of the shell quartet (bra | ket), slice (sa, sc):

@@ -182,10 +182,14 @@ HFX_HD HFX_INLINE void grad_pass(const double* c00, const double* d00, R b10, R 
 }
 
 // g[0..8] += sum_{abcd in slice} gam_abcd d(ab|cd)/d(A, B, C) for the quartet.
-template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
-HFX_HD HFX_INLINE void grad_quartet(const ShellPair& bra, const ShellPair& ket, const PrimPair* __restrict__ prims,
-                                    const PrimExp* __restrict__ pexp, int sa, int sc, const RysTables& tab,
-                                    const R (&gam)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], double (&g)[9]) {
+// bra: shell pair (primitive pairs prims[bra.p0 ..], exponents pexp[bra.p0 ..]);
+// ket: C - D = cd[], primitive pairs and C exponents from kp (outer loop, one
+// per-lane load each; the bra's are the same for the whole wavefront).
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R, class KP>
+HFX_HD HFX_INLINE void grad_quartet_k(const ShellPair& bra, const PrimPair* __restrict__ prims,
+                                      const PrimExp* __restrict__ pexp, double cdx, double cdy, double cdz,
+                                      const KP& kp, int sa, int sc, const RysTables& tab,
+                                      const R (&gam)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], double (&g)[9]) {
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
     int sel_a[3] = {0, 0, 0}, sel_c[3] = {0, 0, 0};
@@ -208,30 +212,32 @@ HFX_HD HFX_INLINE void grad_quartet(const ShellPair& bra, const ShellPair& ket, 
             }
     }
     const R abv[3] = {R(bra.ABx), R(bra.ABy), R(bra.ABz)};
-    const R cdv[3] = {R(ket.ABx), R(ket.ABy), R(ket.ABz)};
+    const R cdv[3] = {R(cdx), R(cdy), R(cdz)};
     R gl[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) gl[k] = R(0);
-    for (int i1 = 0; i1 < bra.np; i1++) {
-        const PrimPair P = prims[bra.p0 + i1];
-        const PrimExp EB = pexp[bra.p0 + i1];
-        for (int k1 = 0; k1 < ket.np; k1++) {
-            const PrimPair Qp = prims[ket.p0 + k1];
-            const PrimExp EK = pexp[ket.p0 + k1];
-            const double p = P.p, q = Qp.p, pq = p + q;
-            const double ip = P.ip, iq = Qp.ip;
+    const int nkp = kp.count();
+    for (int k1 = 0; k1 < nkp; k1++) {
+        const auto Qp = kp.get(k1);
+        const R two_c = R(2.0 * double(kp.cexp(k1)));
+        for (int i1 = 0; i1 < bra.np; i1++) {
+            const PrimPair P = hfx_ld_prim(prims + bra.p0 + i1);
+            const PrimExp EB = hfx_ld_pexp(pexp + bra.p0 + i1);
+            const double p = P.p, q = double(Qp.p), pq = p + q;
+            const double ip = P.ip, iq = double(Qp.ip);
             const double isq = hfx_rsqrt(pq);
             const double ipq = isq * isq;
             const double rho = p * q * ipq;
-            const double PQ[3] = {P.Px - Qp.Px, P.Py - Qp.Py, P.Pz - Qp.Pz};
+            const double PQ[3] = {P.Px - double(Qp.Px), P.Py - double(Qp.Py), P.Pz - double(Qp.Pz)};
             const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-            const double pref = HFX_TWO_PI_52 * P.K * Qp.K * ip * iq * isq;
+            const double pref = HFX_TWO_PI_52 * P.K * double(Qp.K) * ip * iq * isq;
             double u[NR], w[NR];
             if constexpr (NR <= 2)
                 rys_roots_small<NR>(T, u, w, tab.boys);
             else
                 rys_roots<NR>(T, u, w, tab);
-            const double PA[3] = {P.PAx, P.PAy, P.PAz}, QC[3] = {Qp.PAx, Qp.PAy, Qp.PAz};
+            const double PA[3] = {P.PAx, P.PAy, P.PAz};
+            const double QC[3] = {double(Qp.QCx), double(Qp.QCy), double(Qp.QCz)};
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const double ur = u[r];
@@ -248,12 +254,24 @@ HFX_HD HFX_INLINE void grad_quartet(const ShellPair& bra, const ShellPair& ket, 
                 grad_pass<LA, LB, LC, LD, SPA, SPC, 1, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
                                                           R(2.0 * EB.ab), gam, gl);
                 grad_pass<LA, LB, LC, LD, SPA, SPC, 2, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
-                                                          R(2.0 * EK.aa), gam, gl);
+                                                          two_c, gam, gl);
             }
         }
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) g[k] += double(gl[k]);
+}
+
+// The same for a ket given as a shell pair with PrimPair / PrimExp records.
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R>
+HFX_HD HFX_INLINE void grad_quartet(const ShellPair& bra, const ShellPair& ket, const PrimPair* __restrict__ prims,
+                                    const PrimExp* __restrict__ pexp, int sa, int sc, const RysTables& tab,
+                                    const R (&gam)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], double (&g)[9]) {
+    KetPrimsAoSG kp;
+    kp.pp = prims + ket.p0;
+    kp.np = ket.np;
+    kp.pe = pexp + ket.p0;
+    grad_quartet_k<LA, LB, LC, LD, SPA, SPC, R>(bra, prims, pexp, ket.ABx, ket.ABy, ket.ABz, kp, sa, sc, tab, gam, g);
 }
 
 // gam_abcd = s (4 D_ab D_cd - D_ac D_bd - D_ad D_bc) for the thread's slice;

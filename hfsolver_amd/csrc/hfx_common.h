@@ -61,6 +61,11 @@ struct PrimPair {
     double pad;
 };
 
+// Exponents of a primitive pair (used by the derivative integrals).
+struct PrimExp {
+    double aa, ab;
+};
+
 static_assert(sizeof(ShellPair) == 80 && sizeof(PrimPair) == 80, "record layouts read field by field below");
 
 // Loads of read-only data whose address is the same for the whole wavefront:
@@ -98,6 +103,15 @@ HFX_HD HFX_INLINE PrimPair hfx_ld_prim(const PrimPair* p) {
 #endif
 }
 
+HFX_HD HFX_INLINE PrimExp hfx_ld_pexp(const PrimExp* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const HFX_AS4 double* d = (const HFX_AS4 double*)p;
+    return PrimExp{d[0], d[1]};
+#else
+    return *p;
+#endif
+}
+
 HFX_HD HFX_INLINE ShellPair hfx_ld_pair(const ShellPair* p) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const HFX_AS4 int* q = (const HFX_AS4 int*)p;
@@ -123,13 +137,10 @@ HFX_HD HFX_INLINE ShellPair hfx_ld_pair(const ShellPair* p) {
 #endif
 }
 
-// Fields of a ket primitive pair in a field-major ket table: p, 1/p, P, P - C, K.
-static constexpr int KETPRIM_FIELDS = 9;
+// Fields of a ket primitive pair in a field-major ket table: p, 1/p, P, P - C,
+// K, and the exponent of the C primitive (derivative integrals).
+static constexpr int KETPRIM_FIELDS = 10;
 
-// Exponents of a primitive pair (used by the derivative integrals).
-struct PrimExp {
-    double aa, ab;
-};
 
 static constexpr double HFX_PI = 3.14159265358979323846;
 static constexpr double HFX_SQRT_PI = 1.77245385090551602730;

@@ -136,6 +136,7 @@ struct hfx_engine {
     double* d_J = nullptr;
     double* d_K = nullptr;
     int* d_shell_atom = nullptr;
+    int* d_bf_atom = nullptr;
     double* d_grad = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void*> geom_allocs;   // freed on geometry change
@@ -465,9 +466,10 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
             cd[(size_t)nk + k] = sp.ABy;
             cd[2 * (size_t)nk + k] = sp.ABz;
             for (int i = 0; i < t.np[s]; i++) {   // padding: last primitive pair with K = 0
-                const PrimPair& q = h->prims[sp.p0 + std::min(i, sp.np - 1)];
+                const int pi = sp.p0 + std::min(i, sp.np - 1);
+                const PrimPair& q = h->prims[pi];
                 const double K = i < sp.np ? q.K : 0.0;
-                const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, K};
+                const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, K, h->pexp[pi].aa};
                 for (int fi = 0; fi < KETPRIM_FIELDS; fi++)
                     prim[(size_t)t.psoff[s] + ((size_t)i * KETPRIM_FIELDS + fi) * nks + j] = f[fi];
             }
@@ -969,6 +971,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
         std::vector<int> sat(h->shells.size());
         for (size_t i = 0; i < h->shells.size(); i++) sat[i] = h->shells[i].atom;
         if (upload(h, &h->d_shell_atom, sat, false)) return 1;
+        if (upload(h, &h->d_bf_atom, h->bf_atom, false)) return 1;
         if (dmalloc(h, (void**)&h->d_grad, 3 * (size_t)nat * sizeof(double), false)) return 1;
     }
     if (dmalloc(h, (void**)&h->d_D, nn * sizeof(double), false)) return 1;
@@ -1184,6 +1187,46 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     HIPCHK(h, hipMemcpy(h->d_P, P, nn * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(h, hfx_launch_prep(h->d_P, h->d_scale, h->d_D, h->d_Jb, h->d_Kb, n, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_grad, 0, 3 * (size_t)h->nat * sizeof(double), h->stream));
+    // row mode over the ket tables where the J/K launch is in row mode
+    for (int c = 0; c < HFX_NPAIRCLASS; c++)
+        if (h->ktab[c].t.nk > 0)
+            HIPCHK(h, hfx_launch_ket_gather(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_D, n, h->stream));
+    bool done[HFX_NPAIRCLASS][HFX_NPAIRCLASS] = {};
+    for (const auto& c : h->rcombos) {
+        int nrow = 0, nab = 0;
+        hfx_row_shape(c.cb, c.ck, &nrow, &nab);
+        int dlds = c.dlds;
+        if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) dlds = 0;
+        if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) continue;
+        GradRowArgs r;
+        r.pairs = h->d_pairs;
+        r.prims = h->d_prims;
+        r.pexp = h->d_pexp;
+        r.bra_list = h->d_cls_list[c.cb];
+        r.kt = h->ktab[c.ck].t;
+        r.R = c.R;
+        r.nbra = (int)h->cls_list[c.cb].size();
+        r.ncols = c.ncols;
+        r.offd = c.offd;
+        r.bf_c = h->d_bfL[pair_class_la(c.ck)];
+        r.bf_d = h->d_bfL[pair_class_lb(c.ck)];
+        r.dlds = dlds;
+        r.bf_atom = h->d_bf_atom;
+        r.nat = h->nat;
+        r.D = h->d_D;
+        r.grad = h->d_grad;
+        r.n = n;
+        for (int prec = 0; prec < 2; prec++) {
+            if (c.nblk[prec] == 0) continue;
+            r.kbeg = c.d_kbeg[prec];
+            r.kend = c.d_kend[prec];
+            r.blk_row = c.d_blk_row[prec];
+            r.blk_k0 = c.d_blk_k0[prec];
+            r.chunk = c.chunk[prec];
+            HIPCHK(h, hfx_launch_grad_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], h->stream));
+        }
+        done[c.cb][c.ck] = true;
+    }
     GradArgs a;
     a.pairs = h->d_pairs;
     a.prims = h->d_prims;
@@ -1194,6 +1237,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     a.grad = h->d_grad;
     a.n = n;
     for (const auto& c : h->combos) {
+        if (done[c.cb][c.ck]) continue;
         a.bra_list = h->d_cls_list[c.cb];
         a.ket_list = c.d_ket;
         for (int prec = 0; prec < 2; prec++) {
@@ -1418,7 +1462,8 @@ void hfx_destroy(hfx_t* h) {
     free_geometry(h);
     for (int L = 0; L <= HFX_LMAX; L++)
         if (h->d_bfL[L]) (void)hipFree(h->d_bfL[L]);
-    void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K, h->d_shell_atom, h->d_grad};
+    void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K, h->d_shell_atom, h->d_bf_atom,
+                    h->d_grad};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);

@@ -127,6 +127,36 @@ struct GradArgs {
     int n;
 };
 
+// Arguments of one row-mode gradient launch: the blocks, rows, ket table and
+// LDS density rows of the row-mode J/K launch of the same class combination
+// (RowArgs); the nine derivative components of every quartet go to the atoms
+// of its shells (bf_atom of their first functions), summed in LDS per block
+// (3 nat doubles after the density rows) and flushed into grad[3 nat].
+struct GradRowArgs {
+    const ShellPair* pairs;
+    const PrimPair* prims;
+    const PrimExp* pexp;
+    const int* bra_list;
+    KetTab kt;
+    const int* kbeg;
+    const int* kend;
+    const int* blk_row;
+    const int* blk_k0;
+    int chunk;
+    int R;
+    int nbra;
+    int ncols;
+    int offd;
+    const int* bf_c;
+    const int* bf_d;
+    int dlds;
+    const int* bf_atom;
+    int nat;
+    const double* D;
+    double* grad;
+    int n;
+};
+
 // Arguments of the explicit-quartet kernel: quartet q uses pairs bra[q], ket[q]
 // (already oriented l_A >= l_B, l_C >= l_D, class(bra) >= class(ket)) and
 // writes its block at out + off[q] in the (a, b, c, d) order of the oriented
@@ -156,6 +186,17 @@ hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, i
 void hfx_row_shape(int cls_bra, int cls_ket, int* nrow, int* nab);
 hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, const RowArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad(int cls_bra, int cls_ket, bool fp32, const GradArgs& a, int nblocks, hipStream_t s);
+// LDS bytes of a row-mode gradient launch (density rows as in hfx_row_lds, then 3 nat doubles)
+inline size_t hfx_grad_row_lds(int nrow, int ncols, int R, int dlds, bool fp32, int nat) {
+    size_t b = 0;
+    if (dlds) {
+        const size_t m = (size_t)R * nrow * ncols;
+        b += fp32 ? ((m + 1) / 2) * sizeof(double) : m * sizeof(double);
+    }
+    return b + 3 * (size_t)nat * sizeof(double);
+}
+hipError_t hfx_launch_grad_rows(int cls_bra, int cls_ket, bool fp32, const GradRowArgs& a, int nblocks,
+                                hipStream_t s);
 hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
 // Q2[pair] = max over the diagonal integrals (ab|ab) of the pair (squared bound)
 hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,

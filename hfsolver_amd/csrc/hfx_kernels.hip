@@ -317,6 +317,169 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
 #endif
 
 #if HFX_PART == 4 || HFX_PART == 5
+// Two-electron gradient, row mode: the blocks and ket tables of the row-mode
+// J/K launch (see GradRowArgs). Per quartet gam = s (4 D_ab D_cd - D_ac D_bd -
+// D_ad D_bc); A and B components reduced over the wavefront per row, C and D
+// (= -(A + B + C), translational invariance) by LDS atomics on their atoms.
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void grad_row_kernel(GradRowArgs a) {
+    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
+    constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
+    constexpr int nrow = NAt + NB;
+    extern __shared__ double smem[];
+    __shared__ int rbf_s[HFX_ROWS_MAX * nrow];
+    const int ncols = a.ncols;
+    const int row0 = hfx_ld_int(a.blk_row + blockIdx.x);
+    const int nr = min(a.R, a.nbra - row0);
+    const int ndl = a.dlds ? a.R * nrow * ncols : 0;
+    R* Dl = reinterpret_cast<R*>(smem);
+    double* gl = smem + (sizeof(R) == 8 ? ndl : (ndl + 1) / 2);   // 3 * nat
+    const int split = blockIdx.y;
+    const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
+    const int sc = SPC ? split % Q::NC : 0;
+    const int n = a.n;
+    if (threadIdx.x < nr * nrow) {
+        const int r = threadIdx.x / nrow, i = threadIdx.x - r * nrow;
+        const ShellPair bra = a.pairs[a.bra_list[row0 + r]];
+        rbf_s[threadIdx.x] = i < NAt ? bra.fa + sa + i : bra.fb + (i - NAt);
+    }
+    for (int k = threadIdx.x; k < 3 * a.nat; k += blockDim.x) gl[k] = 0.0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nr * nrow * ncols && ndl > 0; k += blockDim.x) {
+        const int ri = k / ncols, col = k - ri * ncols;
+        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[col - a.offd] : a.bf_c[col];
+        Dl[k] = R(a.D[(size_t)rbf_s[ri] * n + cbf]);
+    }
+    __syncthreads();
+    const KetTab& kt = a.kt;
+    const int nk = kt.nk;
+    const int k0 = hfx_ld_int(a.blk_k0 + blockIdx.x), k1 = k0 + a.chunk;
+    const bool lead = (threadIdx.x % 64) == 0;
+    for (int s = 0; s < kt.S; s++) {
+        int lo = 1 << 30, hi = 0;
+        for (int r = 0; r < nr; r++) {
+            lo = min(lo, hfx_ld_int(a.kbeg + (row0 + r) * kt.S + s));
+            hi = max(hi, hfx_ld_int(a.kend + (row0 + r) * kt.S + s));
+        }
+        const int j0 = max(k0, lo), j1 = min(k1, hi);
+        if (j0 >= j1) continue;
+        const int soff = kt.off[s], nks = kt.off[s + 1] - soff, npk = kt.np[s];
+        const double* sprim = kt.prim + kt.psoff[s];
+        for (int base = j0; base < j1; base += (int)blockDim.x) {
+            const int col = base + (int)threadIdx.x;
+            const bool incol = col < j1;
+            KetInfo ki{0, 0, 0, 0};
+            double cdx = 0, cdy = 0, cdz = 0;
+            R Dcd[NCt][ND];
+            int atc = 0, atd = 0;
+            if (incol) {
+                ki = kt.info[col];
+                cdx = kt.cd[col];
+                cdy = kt.cd[nk + col];
+                cdz = kt.cd[2 * nk + col];
+#pragma unroll
+                for (int c = 0; c < NCt; c++)
+#pragma unroll
+                    for (int d = 0; d < ND; d++) Dcd[c][d] = R(kt.Dcd[(size_t)((sc + c) * ND + d) * nk + col]);
+                atc = a.bf_atom[ki.fa];
+                atd = a.bf_atom[ki.fb];
+            }
+            const int cc = (ki.cp & 0x7fff) + sc, cd = a.offd + ((ki.cp >> 15) & 0x7fff);
+            const double sket = (ki.cp >> 30) & 1 ? 0.5 : 1.0;
+            const KetPrimsSoA<double> kp{sprim + (col - soff), npk, nks};
+            double gcd[6] = {0, 0, 0, 0, 0, 0};   // C, D components summed over the rows
+            for (int r = 0; r < nr; r++) {
+                const int row = row0 + r;
+                const int bp = hfx_ld_int(a.bra_list + row);
+                const ShellPair bra = hfx_ld_pair(a.pairs + bp);
+                const bool act = incol && col >= hfx_ld_int(a.kbeg + row * kt.S + s) &&
+                                 col < hfx_ld_int(a.kend + row * kt.S + s);
+                const int fa = bra.fa + sa, fb = bra.fb;
+                double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                if (act) {
+                    double sf = bra.A == bra.B ? 0.5 * sket : sket;
+                    if (bp == ki.pid) sf *= 0.5;
+                    R Dab[NAt][NB], Dac[NAt][NCt], Dad[NAt][ND], Dbc[NB][NCt], Dbd[NB][ND];
+#pragma unroll
+                    for (int i = 0; i < NAt; i++)
+#pragma unroll
+                        for (int j = 0; j < NB; j++) Dab[i][j] = R(a.D[(size_t)(fa + i) * n + fb + j]);
+                    if (ndl > 0) {
+                        const R* Dr = Dl + r * nrow * ncols;
+#pragma unroll
+                        for (int i = 0; i < NAt; i++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) Dac[i][c] = Dr[i * ncols + cc + c];
+#pragma unroll
+                            for (int d = 0; d < ND; d++) Dad[i][d] = Dr[i * ncols + cd + d];
+                        }
+#pragma unroll
+                        for (int b = 0; b < NB; b++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) Dbc[b][c] = Dr[(NAt + b) * ncols + cc + c];
+#pragma unroll
+                            for (int d = 0; d < ND; d++) Dbd[b][d] = Dr[(NAt + b) * ncols + cd + d];
+                        }
+                    } else {
+                        const int fc = ki.fa + sc, fd = ki.fb;
+#pragma unroll
+                        for (int i = 0; i < NAt; i++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) Dac[i][c] = R(a.D[(size_t)(fa + i) * n + fc + c]);
+#pragma unroll
+                            for (int d = 0; d < ND; d++) Dad[i][d] = R(a.D[(size_t)(fa + i) * n + fd + d]);
+                        }
+#pragma unroll
+                        for (int b = 0; b < NB; b++) {
+#pragma unroll
+                            for (int c = 0; c < NCt; c++) Dbc[b][c] = R(a.D[(size_t)(fb + b) * n + fc + c]);
+#pragma unroll
+                            for (int d = 0; d < ND; d++) Dbd[b][d] = R(a.D[(size_t)(fb + b) * n + fd + d]);
+                        }
+                    }
+                    R gam[Q::NOUT];
+                    const R rs = R(sf);
+#pragma unroll
+                    for (int i = 0; i < NAt; i++)
+#pragma unroll
+                        for (int b = 0; b < NB; b++)
+#pragma unroll
+                            for (int c = 0; c < NCt; c++)
+#pragma unroll
+                                for (int d = 0; d < ND; d++)
+                                    gam[((i * NB + b) * NCt + c) * ND + d] =
+                                        rs * (R(4) * Dab[i][b] * Dcd[c][d] - Dac[i][c] * Dbd[b][d] - Dad[i][d] * Dbc[b][c]);
+                    grad_quartet_k<LA, LB, LC, LD, SPA, SPC, R>(bra, a.prims, a.pexp, cdx, cdy, cdz, kp, sa, sc,
+                                                                dev_tables(), gam, g);
+#pragma unroll
+                    for (int x = 0; x < 3; x++) {
+                        gcd[x] += g[6 + x];
+                        gcd[3 + x] -= g[x] + g[3 + x] + g[6 + x];
+                    }
+                }
+                // A and B of row r: reduce over the wavefront, one LDS add per wavefront
+                const int ata = a.bf_atom[bra.fa], atb = a.bf_atom[bra.fb];
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    const double v = wave_sum(g[k]);
+                    if (lead && v != 0.0) atomicAdd(&gl[3 * (k < 3 ? ata : atb) + k % 3], v);
+                }
+            }
+            if (incol) {
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    if (gcd[x] != 0.0) atomicAdd(&gl[3 * atc + x], gcd[x]);
+                    if (gcd[3 + x] != 0.0) atomicAdd(&gl[3 * atd + x], gcd[3 + x]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * a.nat; k += blockDim.x)
+        if (gl[k] != 0.0) unsafeAtomicAdd(&a.grad[k], gl[k]);
+}
+
 // Two-electron gradient: one block = one bra shell pair x a chunk of its kets.
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
@@ -580,6 +743,27 @@ hipError_t grad_xy(const GradArgs& a, int nblocks, hipStream_t s) {
 }
 typedef hipError_t (*grad_fn)(const GradArgs&, int, hipStream_t);
 const grad_fn GRAD_TABLE[6][6] = HFX_TABLE(grad_xy);
+
+template <int X, int Y>
+hipError_t grad_rows_xy(const GradRowArgs& a, int nblocks, hipStream_t s) {
+    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
+    constexpr int NB = ncart(pair_class_lb(X));
+    const size_t lds = hfx_grad_row_lds(NAt + NB, a.ncols, a.R, a.dlds, sizeof(GradReal) == 4, a.nat);
+    if (lds > HFX_ROW_LDS_MAX || a.R < 1 || a.R > HFX_ROWS_MAX || a.kt.S < 1 || a.kt.S > HFX_SUB_MAX)
+        return hipErrorInvalidValue;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)grad_row_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, HFX_ROW_LDS_MAX);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    hipLaunchKernelGGL((grad_row_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>), grid, dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+typedef hipError_t (*grad_rows_fn)(const GradRowArgs&, int, hipStream_t);
+const grad_rows_fn GRAD_ROWS_TABLE[6][6] = HFX_TABLE(grad_rows_xy);
 #endif
 
 #if HFX_PART == 6
@@ -630,11 +814,19 @@ hipError_t hfx_launch_grad_f64(int cb, int ck, const GradArgs& a, int nblocks, h
     HFX_CHECK_COMBO(cb, ck);
     return GRAD_TABLE[cb][ck](a, nblocks, s);
 }
+hipError_t hfx_launch_grad_rows_f64(int cb, int ck, const GradRowArgs& a, int nblocks, hipStream_t s) {
+    HFX_CHECK_COMBO(cb, ck);
+    return GRAD_ROWS_TABLE[cb][ck](a, nblocks, s);
+}
 #endif
 #if HFX_PART == 5
 hipError_t hfx_launch_grad_f32(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s) {
     HFX_CHECK_COMBO(cb, ck);
     return GRAD_TABLE[cb][ck](a, nblocks, s);
+}
+hipError_t hfx_launch_grad_rows_f32(int cb, int ck, const GradRowArgs& a, int nblocks, hipStream_t s) {
+    HFX_CHECK_COMBO(cb, ck);
+    return GRAD_ROWS_TABLE[cb][ck](a, nblocks, s);
 }
 #endif
 
@@ -643,6 +835,8 @@ hipError_t hfx_launch_jk_rows_f64(int cb, int ck, const RowArgs& a, int nblocks,
 hipError_t hfx_launch_jk_rows_f32(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad_f64(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad_f32(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_grad_rows_f64(int cb, int ck, const GradRowArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_grad_rows_f32(int cb, int ck, const GradRowArgs& a, int nblocks, hipStream_t s);
 
 int hfx_nsplit(int cb, int ck) {
     return nsplit(pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck));
@@ -662,6 +856,11 @@ hipError_t hfx_launch_jk_rows(int cb, int ck, bool fp32, const RowArgs& a, int n
 hipError_t hfx_launch_grad(int cb, int ck, bool fp32, const GradArgs& a, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipErrorInvalidValue;
     return fp32 ? hfx_launch_grad_f32(cb, ck, a, nblocks, s) : hfx_launch_grad_f64(cb, ck, a, nblocks, s);
+}
+
+hipError_t hfx_launch_grad_rows(int cb, int ck, bool fp32, const GradRowArgs& a, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipErrorInvalidValue;
+    return fp32 ? hfx_launch_grad_rows_f32(cb, ck, a, nblocks, s) : hfx_launch_grad_rows_f64(cb, ck, a, nblocks, s);
 }
 
 hipError_t hfx_launch_block(int cb, int ck, const BlockArgs& a, hipStream_t s) {
