@@ -36,7 +36,7 @@ def build_library(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     # the kernel source builds in parts (see its header), in parallel
     parts = {1: [], 2: ["-DHFX_REAL_T=double"], 3: ["-DHFX_REAL_T=float"], 4: ["-DHFX_REAL_T=double"],
-             5: ["-DHFX_REAL_T=float"], 6: []}
+             5: ["-DHFX_REAL_T=float"], 6: [], 7: []}
     objs = []
     jobs = []
     for part, extra in parts.items():

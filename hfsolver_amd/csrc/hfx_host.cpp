@@ -5,14 +5,19 @@
 //      into Cartesian shells, as src/gaussians_libint.f90:59-122 recovers shells
 //      by stepping (L+1)(L+2)/2 functions; each function's contraction
 //      coefficients are (shell coefficients) x (per-function scale);
-//   2. all shell pairs (A,B), oriented l_A >= l_B, with their primitive pairs;
+//   2. all shell pairs (A,B), oriented l_A >= l_B, with their primitive pairs
+//      (negligible primitive pairs screened out);
 //   3. Schwarz bounds Q_AB = sqrt(max |(ab|ab)|) on the GPU;
-//   4. per pair class, the significant pairs sorted by Q (descending), and for
-//      every class combination the 8x32 tiles of the (bra, ket) plane whose
-//      quartets pass Q_ab Q_cd >= schwarz_threshold, split into an FP64 and an
-//      FP32 share at fp32_threshold.
-// Per Fock build (hfx_fock_2e): P -> scaled symmetric D, one launch per class
-// combination and precision, then J, K symmetrised and unscaled, G = J - K/2.
+//   4. per pair class, the significant pairs sorted by Q (descending) and the
+//      class's field-major ket table (sublists by primitive count); for every
+//      class combination each bra row's range of kets per sublist with
+//      Q_ab Q_cd >= schwarz_threshold, split into an FP64 and an FP32 share at
+//      fp32_threshold, and the row-mode block list (tiles where the LDS rows do
+//      not fit).
+// Per Fock build (hfx_fock_2e): P -> scaled symmetric D, D_cd gathered per ket,
+// one launch per class combination and precision, J_cd scattered, then J, K
+// symmetrised and unscaled, G = J - K/2. One-electron integrals and their
+// gradient, the two-electron gradient and the store mode run on the GPU too.
 #include <hip/hip_runtime.h>
 #include <omp.h>
 
@@ -31,12 +36,6 @@
 #include "hfx_common.h"
 #include "hfx_internal.h"
 #include "rys.h"
-
-#define RYS_TABLE_QUALIFIER static const
-namespace hfx_host_tables {
-#include "rys_tables.inc"
-}
-#undef RYS_TABLE_QUALIFIER
 
 namespace {
 
@@ -137,6 +136,16 @@ struct hfx_engine {
     double* d_K = nullptr;
     int* d_shell_atom = nullptr;
     int* d_bf_atom = nullptr;
+    // one-electron kernels: shell data, nuclei, shell-pair lists [ordered][l_A][l_B]
+    struct OneEList {
+        int* A = nullptr;
+        int* B = nullptr;
+        int n = 0;
+    };
+    OneEList oel[2][HFX_LMAX + 1][HFX_LMAX + 1];
+    int *d_sh_atom = nullptr, *d_sh_f0 = nullptr, *d_sh_nprim = nullptr, *d_sh_prim0 = nullptr;
+    double *d_alpha = nullptr, *d_coef = nullptr, *d_Z = nullptr;
+    double* d_xyz = nullptr;   // 3 nat, geometry
     double* d_grad = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void*> geom_allocs;   // freed on geometry change
@@ -187,22 +196,6 @@ void free_geometry(hfx_t* h) {
     for (int c = 0; c < HFX_NPAIRCLASS; c++) {
         h->d_cls_list[c] = nullptr;
         h->cls_list[c].clear();
-    }
-}
-
-RysTables host_tables() {
-    return RysTables{hfx_host_tables::RYS_CHEB, hfx_host_tables::RYS_HERM_U, hfx_host_tables::RYS_HERM_W,
-                     hfx_host_tables::BOYS_TAB, hfx_host_tables::BOYSF_TAB, hfx_host_tables::RYS_CHEBF};
-}
-
-void rys_roots_rt(int nr, double T, double* u, double* w) {
-    const RysTables t = host_tables();
-    switch (nr) {
-        case 1: rys_roots<1>(T, u, w, t); break;
-        case 2: rys_roots<2>(T, u, w, t); break;
-        case 3: rys_roots<3>(T, u, w, t); break;
-        case 4: rys_roots<4>(T, u, w, t); break;
-        default: rys_roots<5>(T, u, w, t); break;
     }
 }
 
@@ -734,6 +727,7 @@ int build_lists(hfx_t* h) {
 
 int setup_geometry(hfx_t* h) {
     free_geometry(h);
+    if (upload(h, &h->d_xyz, h->xyz, true)) return 1;
     build_pairs(h);
     if (upload(h, &h->d_pairs, h->pairs, true)) return 1;
     if (upload(h, &h->d_prims, h->prims, true)) return 1;
@@ -744,173 +738,70 @@ int setup_geometry(hfx_t* h) {
 }
 
 // ------------------------------------------------------ one-electron ----
-// Primitive pairs whose bound on every one-electron contribution is below
-// this are skipped (far below the accuracy of the two-electron screening).
-static constexpr double HFX_ONE_E_SCREEN = 1e-17;
-
-// 1-D overlap table S[i][j] (i <= imax, j <= jmax) of a primitive pair, without
-// the exp / pi factors: S(0,0) = 1, Obara-Saika recurrences.
-void os_overlap_1d(double PA, double PB, double oo2p, int imax, int jmax, double S[8][8]) {
-    S[0][0] = 1.0;
-    for (int i = 0; i <= imax; i++)
-        for (int j = 0; j <= jmax; j++) {
-            if (i == 0 && j == 0) continue;
-            double v;
-            if (i > 0) {
-                v = PA * S[i - 1][j];
-                if (i > 1) v += (i - 1) * oo2p * S[i - 2][j];
-                if (j > 0) v += j * oo2p * S[i - 1][j - 1];
-            } else {
-                v = PB * S[i][j - 1];
-                if (j > 1) v += (j - 1) * oo2p * S[i][j - 2];
-            }
-            S[i][j] = v;
-        }
-}
-
-// Nuclear-attraction 2-D table for one Rys root: I[i][j] for i <= imax, j <= jmax.
-void rys_nuc_1d(double c00, double b10, double AB, int imax, int jmax, double g0, double I[8][8]) {
-    double G[16];
-    const int nmax = imax + jmax;
-    G[0] = g0;
-    if (nmax > 0) G[1] = c00 * g0;
-    for (int k = 1; k < nmax; k++) G[k + 1] = c00 * G[k] + k * b10 * G[k - 1];
-    double H[16][8];
-    for (int k = 0; k <= nmax; k++) H[k][0] = G[k];
-    for (int j = 1; j <= jmax; j++)
-        for (int k = 0; k <= nmax - j; k++) H[k][j] = H[k + 1][j - 1] + AB * H[k][j - 1];
-    for (int i = 0; i <= imax; i++)
-        for (int j = 0; j <= jmax; j++) I[i][j] = H[i][j];
-}
-
-// Primitive-pair blocks of S, T, V (and optionally their derivatives with
-// respect to the centre of A) for shells A, B. da: derivative order 0 or 1.
-struct OneEBlock {
-    double S[6][6], T[6][6], V[6][6];
-    double dS[3][6][6], dT[3][6][6], dV[3][6][6];  // d/dA
-};
-
-// With deriv, pw (density block of A x B, scaled) and gop (3*nat) given, the
-// operator (Hellmann-Feynman) part -2 sum_ab P_ab d/dA <a|V_c|b> is added to
-// gop[3c + d] (d/dC = -(d/dA + d/dB); the d/dB half is the (B, A) visit's).
-void one_e_block(const hfx_t* h, int A, int B, bool deriv, OneEBlock& out, const double (*pw)[6] = nullptr,
-                 double* gop = nullptr) {
-    const Shell& sa = h->shells[A];
-    const Shell& sb = h->shells[B];
-    const int la = sa.L, lb = sb.L, na = ncart(la), nb = ncart(lb);
-    const double* ra = &h->xyz[3 * sa.atom];
-    const double* rb = &h->xyz[3 * sb.atom];
-    const double AB[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
-    const double r2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
-    std::memset(out.S, 0, sizeof out.S);
-    std::memset(out.T, 0, sizeof out.T);
-    std::memset(out.V, 0, sizeof out.V);
-    std::memset(out.dS, 0, sizeof out.dS);
-    std::memset(out.dT, 0, sizeof out.dT);
-    std::memset(out.dV, 0, sizeof out.dV);
-    const int ea = la + (deriv ? 1 : 0);  // highest power needed on A
-    // bound on the primitive pair's contributions (overlap, kinetic and all
-    // nuclei, with their derivatives): |K| (pi/p)^{3/2} or |K| 2 pi/p sum Z,
-    // times (1 + a + b)^2 for the kinetic / derivative factors and
-    // (1 + |A-B|)^(la + lb + 1) for the Cartesian polynomials
-    const double poly = std::pow(1.0 + std::sqrt(r2), la + lb + 1);
-    for (int pa = 0; pa < sa.nprim; pa++)
-        for (int pb = 0; pb < sb.nprim; pb++) {
-            const double a = h->alpha[sa.prim0 + pa], b = h->alpha[sb.prim0 + pb];
-            const double p = a + b, oo2p = 0.5 / p;
-            const double mu_r2 = a * b / p * r2;
-            if (mu_r2 > 745.0) continue;   // exp underflows to 0
-            const double K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-mu_r2);
-            const double bound = std::fabs(K) * std::max(std::pow(HFX_PI / p, 1.5), 2.0 * HFX_PI / p * h->ztot) *
-                                 (1.0 + a + b) * (1.0 + a + b) * poly;
-            if (bound < HFX_ONE_E_SCREEN) continue;
-            double P[3], PA[3], PB[3];
-            for (int d = 0; d < 3; d++) {
-                P[d] = (a * ra[d] + b * rb[d]) / p;
-                PA[d] = P[d] - ra[d];
-                PB[d] = P[d] - rb[d];
-            }
-            // overlap / kinetic
-            double S1[3][8][8];
-            for (int d = 0; d < 3; d++) os_overlap_1d(PA[d], PB[d], oo2p, ea, lb + 2, S1[d]);
-            const double s0 = std::pow(HFX_PI / p, 1.5) * K;
-            auto Tk = [&](int d, int i, int j) {  // 1-D kinetic (-1/2 d2/dx2 on b)
-                double v = -2.0 * b * b * S1[d][i][j + 2] + b * (2 * j + 1) * S1[d][i][j];
-                if (j >= 2) v -= 0.5 * j * (j - 1) * S1[d][i][j - 2];
-                return v;
-            };
-            auto ST = [&](int ax, int ay, int az, int bx, int by, int bz, double& s, double& t) {
-                if (ax < 0 || ay < 0 || az < 0) {
-                    s = t = 0;
-                    return;
-                }
-                const double sx = S1[0][ax][bx], sy = S1[1][ay][by], sz = S1[2][az][bz];
-                s = s0 * sx * sy * sz;
-                t = s0 * (Tk(0, ax, bx) * sy * sz + sx * Tk(1, ay, by) * sz + sx * sy * Tk(2, az, bz));
-            };
-            for (int ia = 0; ia < na; ia++) {
-                const int ax = cart_exp(la, ia, 0), ay = cart_exp(la, ia, 1), az = cart_exp(la, ia, 2);
-                for (int ib = 0; ib < nb; ib++) {
-                    const int bx = cart_exp(lb, ib, 0), by = cart_exp(lb, ib, 1), bz = cart_exp(lb, ib, 2);
-                    double s, t;
-                    ST(ax, ay, az, bx, by, bz, s, t);
-                    out.S[ia][ib] += s;
-                    out.T[ia][ib] += t;
-                    if (deriv) {
-                        const int e[3] = {ax, ay, az};
-                        for (int d = 0; d < 3; d++) {
-                            int up[3] = {ax, ay, az}, dn[3] = {ax, ay, az};
-                            up[d]++;
-                            dn[d]--;
-                            double su, tu, sd = 0, td = 0;
-                            ST(up[0], up[1], up[2], bx, by, bz, su, tu);
-                            if (e[d] > 0) ST(dn[0], dn[1], dn[2], bx, by, bz, sd, td);
-                            out.dS[d][ia][ib] += 2 * a * su - e[d] * sd;
-                            out.dT[d][ia][ib] += 2 * a * tu - e[d] * td;
+// Shell pairs of every (l_A, l_B): unordered (A >= B, the integrals) and
+// ordered (the gradient, derivative on the centre of A), geometry-independent.
+int build_one_e_lists(hfx_t* h) {
+    const int ns = (int)h->shells.size();
+    for (int la = 0; la <= HFX_LMAX; la++)
+        for (int lb = 0; lb <= HFX_LMAX; lb++)
+            for (int ord = 0; ord < 2; ord++) {
+                std::vector<int> A, B;
+                for (int i = 0; i < ns; i++)
+                    for (int j = 0; j < ns; j++) {
+                        if (ord == 0 && j > i) continue;
+                        if (h->shells[i].L == la && h->shells[j].L == lb) {
+                            A.push_back(i);
+                            B.push_back(j);
                         }
                     }
-                }
+                hfx_engine::OneEList& L = h->oel[ord][la][lb];
+                L.n = (int)A.size();
+                if (L.n == 0) continue;
+                if (upload(h, &L.A, A, false) || upload(h, &L.B, B, false)) return 1;
             }
-            // nuclear attraction by Rys quadrature
-            const int nr = (ea + lb) / 2 + 1;
-            for (int c = 0; c < h->nat; c++) {
-                const double* rc = &h->xyz[3 * c];
-                const double PC[3] = {P[0] - rc[0], P[1] - rc[1], P[2] - rc[2]};
-                const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                double u[RYS_NMAX], w[RYS_NMAX];
-                rys_roots_rt(nr, T, u, w);
-                const double pref = -h->atno[c] * 2.0 * HFX_PI / p * K;
-                for (int r = 0; r < nr; r++) {
-                    double I[3][8][8];
-                    const double b10 = (1.0 - u[r]) * oo2p;
-                    for (int d = 0; d < 3; d++)
-                        rys_nuc_1d(PA[d] - u[r] * PC[d], b10, AB[d], ea, lb, d == 2 ? w[r] * pref : 1.0, I[d]);
-                    auto val = [&](int ax, int ay, int az, int bx, int by, int bz) {
-                        if (ax < 0 || ay < 0 || az < 0) return 0.0;
-                        return I[0][ax][bx] * I[1][ay][by] * I[2][az][bz];
-                    };
-                    for (int ia = 0; ia < na; ia++) {
-                        const int ax = cart_exp(la, ia, 0), ay = cart_exp(la, ia, 1), az = cart_exp(la, ia, 2);
-                        for (int ib = 0; ib < nb; ib++) {
-                            const int bx = cart_exp(lb, ib, 0), by = cart_exp(lb, ib, 1), bz = cart_exp(lb, ib, 2);
-                            out.V[ia][ib] += val(ax, ay, az, bx, by, bz);
-                            if (deriv) {
-                                const int e[3] = {ax, ay, az};
-                                for (int d = 0; d < 3; d++) {
-                                    int up[3] = {ax, ay, az}, dn[3] = {ax, ay, az};
-                                    up[d]++;
-                                    dn[d]--;
-                                    const double dv = 2 * a * val(up[0], up[1], up[2], bx, by, bz) -
-                                                      (e[d] > 0 ? e[d] * val(dn[0], dn[1], dn[2], bx, by, bz) : 0.0);
-                                    out.dV[d][ia][ib] += dv;
-                                    if (gop) gop[3 * c + d] -= 2.0 * pw[ia][ib] * dv;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-        }
+    std::vector<int> sat, sf0, snp, sp0;
+    for (const Shell& sh : h->shells) {
+        sat.push_back(sh.atom);
+        sf0.push_back(sh.f0);
+        snp.push_back(sh.nprim);
+        sp0.push_back(sh.prim0);
+    }
+    std::vector<double> Z(h->atno.begin(), h->atno.end());
+    if (upload(h, &h->d_sh_atom, sat, false) || upload(h, &h->d_sh_f0, sf0, false) ||
+        upload(h, &h->d_sh_nprim, snp, false) || upload(h, &h->d_sh_prim0, sp0, false) ||
+        upload(h, &h->d_alpha, h->alpha, false) || upload(h, &h->d_coef, h->coef, false) ||
+        upload(h, &h->d_Z, Z, false))
+        return 1;
+    return 0;
+}
+
+OneEArgs one_e_args(hfx_t* h) {
+    OneEArgs a{};
+    a.sh_atom = h->d_sh_atom;
+    a.sh_f0 = h->d_sh_f0;
+    a.sh_nprim = h->d_sh_nprim;
+    a.sh_prim0 = h->d_sh_prim0;
+    a.alpha = h->d_alpha;
+    a.coef = h->d_coef;
+    a.xyz = h->d_xyz;
+    a.Z = h->d_Z;
+    a.nat = h->nat;
+    a.ztot = h->ztot;
+    a.scale = h->d_scale;
+    a.n = h->nbf;
+    return a;
+}
+
+// n x n device work buffers of the host-pointer entry points
+int ensure_work(hfx_t* h) {
+    const size_t nn = (size_t)h->nbf * h->nbf;
+    if (!h->d_P) {
+        if (dmalloc(h, (void**)&h->d_P, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_G, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_J, nn * sizeof(double), false)) return 1;
+        if (dmalloc(h, (void**)&h->d_K, nn * sizeof(double), false)) return 1;
+    }
+    return 0;
 }
 
 }  // namespace
@@ -977,6 +868,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (dmalloc(h, (void**)&h->d_D, nn * sizeof(double), false)) return 1;
     if (dmalloc(h, (void**)&h->d_Jb, nn * sizeof(double), false)) return 1;
     if (dmalloc(h, (void**)&h->d_Kb, nn * sizeof(double), false)) return 1;
+    if (build_one_e_lists(h)) return 1;
     return setup_geometry(h);
 }
 
@@ -1086,12 +978,7 @@ int hfx_fock_2e(hfx_t* h, const double* P, double* G, double* J, double* K, hfx_
     if (!h || !P) return h ? fail(h, "hfx_fock_2e: null density") : 1;
     HIPCHK(h, hipSetDevice(h->device));
     const size_t nn = (size_t)h->nbf * h->nbf;
-    if (!h->d_P) {
-        if (dmalloc(h, (void**)&h->d_P, nn * sizeof(double), false)) return 1;
-        if (dmalloc(h, (void**)&h->d_G, nn * sizeof(double), false)) return 1;
-        if (dmalloc(h, (void**)&h->d_J, nn * sizeof(double), false)) return 1;
-        if (dmalloc(h, (void**)&h->d_K, nn * sizeof(double), false)) return 1;
-    }
+    if (ensure_work(h)) return 1;
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(h, hipMemcpy(h->d_P, P, nn * sizeof(double), hipMemcpyHostToDevice));
     if (hfx_fock_2e_device(h, h->d_P, G ? h->d_G : nullptr, J ? h->d_J : nullptr, K ? h->d_K : nullptr, st)) return 1;
@@ -1104,72 +991,58 @@ int hfx_fock_2e(hfx_t* h, const double* P, double* G, double* J, double* K, hfx_
 
 int hfx_one_electron(hfx_t* h, double* S, double* T, double* V) {
     if (!h) return 1;
-    const int ns = (int)h->shells.size();
-    const int n = h->nbf;
-    std::vector<std::pair<int, int>> sp;
-    for (int i = 0; i < ns; i++)
-        for (int j = 0; j <= i; j++) sp.push_back({i, j});
-#pragma omp parallel for schedule(dynamic, 4)
-    for (long k = 0; k < (long)sp.size(); k++) {
-        const int A = sp[k].first, B = sp[k].second;
-        OneEBlock blk;
-        one_e_block(h, A, B, false, blk);
-        const Shell& sa = h->shells[A];
-        const Shell& sb = h->shells[B];
-        for (int ia = 0; ia < ncart(sa.L); ia++)
-            for (int ib = 0; ib < ncart(sb.L); ib++) {
-                const int fi = sa.f0 + ia, fj = sb.f0 + ib;
-                const double sc = h->scale[fi] * h->scale[fj];
-                const size_t ij = (size_t)fi + (size_t)fj * n, ji = (size_t)fj + (size_t)fi * n;
-                if (S) S[ij] = S[ji] = sc * blk.S[ia][ib];
-                if (T) {
-                    // the kinetic operator acts on b; symmetric in exact arithmetic
-                    T[ij] = T[ji] = sc * blk.T[ia][ib];
-                }
-                if (V) V[ij] = V[ji] = sc * blk.V[ia][ib];
-            }
-    }
+    HIPCHK(h, hipSetDevice(h->device));
+    if (ensure_work(h)) return 1;
+    const size_t nn = (size_t)h->nbf * h->nbf;
+    // shell pairs whose every primitive pair is screened out keep zeros
+    HIPCHK(h, hipMemsetAsync(h->d_G, 0, nn * sizeof(double), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_J, 0, nn * sizeof(double), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_K, 0, nn * sizeof(double), h->stream));
+    OneEArgs a = one_e_args(h);
+    a.S = h->d_G;
+    a.T = h->d_J;
+    a.V = h->d_K;
+    for (int la = 0; la <= HFX_LMAX; la++)
+        for (int lb = 0; lb <= HFX_LMAX; lb++) {
+            const hfx_engine::OneEList& L = h->oel[0][la][lb];
+            if (L.n == 0) continue;
+            a.pairA = L.A;
+            a.pairB = L.B;
+            a.npairs = L.n;
+            HIPCHK(h, hfx_launch_one_e(la, lb, false, a, h->stream));
+        }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (S) HIPCHK(h, hipMemcpy(S, h->d_G, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (T) HIPCHK(h, hipMemcpy(T, h->d_J, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (V) HIPCHK(h, hipMemcpy(V, h->d_K, nn * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 
 int hfx_grad_1e(hfx_t* h, const double* P, const double* W, double* g) {
     if (!h || !P || !W || !g) return h ? fail(h, "hfx_grad_1e: null argument") : 1;
-    const int ns = (int)h->shells.size();
-    const int n = h->nbf, nat = h->nat;
-    std::vector<double> acc(3 * nat, 0.0);
-#pragma omp parallel
-    {
-        std::vector<double> gl(3 * nat, 0.0);
-        OneEBlock blk;
-#pragma omp for schedule(dynamic, 4)
-        for (long k = 0; k < (long)ns * ns; k++) {
-            const int A = (int)(k / ns), B = (int)(k % ns);
-            const Shell& sa = h->shells[A];
-            const Shell& sb = h->shells[B];
-            double pw[6][6];
-            for (int ia = 0; ia < ncart(sa.L); ia++)
-                for (int ib = 0; ib < ncart(sb.L); ib++) {
-                    const int fi = sa.f0 + ia, fj = sb.f0 + ib;
-                    pw[ia][ib] = P[(size_t)fi + (size_t)fj * n] * h->scale[fi] * h->scale[fj];
-                }
-            one_e_block(h, A, B, true, blk, pw, gl.data());
-            for (int ia = 0; ia < ncart(sa.L); ia++)
-                for (int ib = 0; ib < ncart(sb.L); ib++) {
-                    const int fi = sa.f0 + ia, fj = sb.f0 + ib;
-                    const double sc = h->scale[fi] * h->scale[fj];
-                    const double pij = P[(size_t)fi + (size_t)fj * n] * sc, wij = W[(size_t)fi + (size_t)fj * n] * sc;
-                    for (int d = 0; d < 3; d++) {
-                        // derivative on the centre of function i; the factor 2 accounts
-                        // for the equal derivative on the centre of j (symmetric P, W)
-                        gl[3 * sa.atom + d] +=
-                            2.0 * (pij * (blk.dT[d][ia][ib] + blk.dV[d][ia][ib]) - wij * blk.dS[d][ia][ib]);
-                    }
-                }
+    HIPCHK(h, hipSetDevice(h->device));
+    if (ensure_work(h)) return 1;
+    if (3 * (size_t)h->nat * sizeof(double) > HFX_ROW_LDS_MAX)
+        return fail(h, "hfx_grad_1e: %d atoms exceed the per-block accumulator", h->nat);
+    const size_t nn = (size_t)h->nbf * h->nbf;
+    HIPCHK(h, hipMemcpy(h->d_P, P, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_G, W, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemsetAsync(h->d_grad, 0, 3 * (size_t)h->nat * sizeof(double), h->stream));
+    OneEArgs a = one_e_args(h);
+    a.P = h->d_P;
+    a.W = h->d_G;
+    a.grad = h->d_grad;
+    for (int la = 0; la <= HFX_LMAX; la++)
+        for (int lb = 0; lb <= HFX_LMAX; lb++) {
+            const hfx_engine::OneEList& L = h->oel[1][la][lb];
+            if (L.n == 0) continue;
+            a.pairA = L.A;
+            a.pairB = L.B;
+            a.npairs = L.n;
+            HIPCHK(h, hfx_launch_one_e(la, lb, true, a, h->stream));
         }
-#pragma omp critical
-        for (int t = 0; t < 3 * nat; t++) acc[t] += gl[t];
-    }
-    for (int t = 0; t < 3 * nat; t++) g[t] = acc[t];
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(g, h->d_grad, 3 * (size_t)h->nat * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1178,12 +1051,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     HIPCHK(h, hipSetDevice(h->device));
     const int n = h->nbf;
     const size_t nn = (size_t)n * n;
-    if (!h->d_P) {
-        if (dmalloc(h, (void**)&h->d_P, nn * sizeof(double), false)) return 1;
-        if (dmalloc(h, (void**)&h->d_G, nn * sizeof(double), false)) return 1;
-        if (dmalloc(h, (void**)&h->d_J, nn * sizeof(double), false)) return 1;
-        if (dmalloc(h, (void**)&h->d_K, nn * sizeof(double), false)) return 1;
-    }
+    if (ensure_work(h)) return 1;
     HIPCHK(h, hipMemcpy(h->d_P, P, nn * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(h, hfx_launch_prep(h->d_P, h->d_scale, h->d_D, h->d_Jb, h->d_Kb, n, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_grad, 0, 3 * (size_t)h->nat * sizeof(double), h->stream));
@@ -1462,8 +1330,14 @@ void hfx_destroy(hfx_t* h) {
     free_geometry(h);
     for (int L = 0; L <= HFX_LMAX; L++)
         if (h->d_bfL[L]) (void)hipFree(h->d_bfL[L]);
+    for (int o = 0; o < 2; o++)
+        for (int la = 0; la <= HFX_LMAX; la++)
+            for (int lb = 0; lb <= HFX_LMAX; lb++) {
+                if (h->oel[o][la][lb].A) (void)hipFree(h->oel[o][la][lb].A);
+                if (h->oel[o][la][lb].B) (void)hipFree(h->oel[o][la][lb].B);
+            }
     void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K, h->d_shell_atom, h->d_bf_atom,
-                    h->d_grad};
+                    h->d_grad, h->d_sh_atom, h->d_sh_f0, h->d_sh_nprim, h->d_sh_prim0, h->d_alpha, h->d_coef, h->d_Z};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);

@@ -157,6 +157,37 @@ struct GradRowArgs {
     int n;
 };
 
+// Arguments of the one-electron kernels: a list of shell pairs (A, B) with
+// l_A = LA, l_B = LB, one wavefront per pair. Without derivatives the S, T, V
+// blocks (and their transposes) are written into column-major n x n matrices;
+// with derivatives the gradient of sum P (T + V) - sum W S with respect to the
+// centre of A (over all ordered pairs) and the Hellmann-Feynman term of every
+// nucleus are summed into grad[3 nat] (3 nat doubles of LDS per block).
+struct OneEArgs {
+    const int* pairA;
+    const int* pairB;
+    int npairs;
+    const int* sh_atom;
+    const int* sh_f0;
+    const int* sh_nprim;
+    const int* sh_prim0;
+    const double* alpha;   // shell-level primitives
+    const double* coef;
+    const double* xyz;     // 3 nat, bohr
+    const double* Z;       // nat
+    int nat;
+    double ztot;           // sum of |Z| (screening bound)
+    const double* scale;   // per function
+    int n;
+    double* S;             // integrals
+    double* T;
+    double* V;
+    const double* P;       // gradient
+    const double* W;
+    double* grad;
+};
+static constexpr double HFX_ONE_E_SCREEN = 1e-17;   // primitive-pair bound below which terms are dropped
+
 // Arguments of the explicit-quartet kernel: quartet q uses pairs bra[q], ket[q]
 // (already oriented l_A >= l_B, l_C >= l_D, class(bra) >= class(ket)) and
 // writes its block at out + off[q] in the (a, b, c, d) order of the oriented
@@ -210,3 +241,4 @@ hipError_t hfx_launch_ket_scatter(const KetTab& kt, int ncomp, int nd, double* J
 hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* scale, double* G, double* J, double* K,
                              int n, hipStream_t s);
 int hfx_nsplit(int cls_bra, int cls_ket);
+hipError_t hfx_launch_one_e(int la, int lb, bool deriv, const OneEArgs& a, hipStream_t s);

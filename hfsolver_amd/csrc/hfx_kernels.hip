@@ -6,18 +6,21 @@
 //                Replaces getInts (src/qc.c:95-132) + formG (src/scf.f90:26-37).
 // schwarz_kernel diagonal (ab|ab) blocks -> Schwarz bounds of every pair.
 // block_kernel   explicit quartets written out (testing / store mode).
+// one_e_kernel   S, T, V (src/qc.c:6-93) and their nuclear gradient.
 // prep / finish  density scaling + symmetrisation, J/K symmetrisation,
 //                G = J - K/2.
 #include <hip/hip_runtime.h>
 
 #include "grad_rys.h"
 #include "hfx_internal.h"
+#include "one_e.h"
 
-// The file is compiled once per part (-DHFX_PART=1..6) so the parts build in
+// The file is compiled once per part (-DHFX_PART=1..7) so the parts build in
 // parallel: 1 tile-mode J/K, 2/3 row-mode J/K in FP64/FP32, 4/5 gradient in
-// FP64/FP32, 6 Schwarz, explicit quartets, prep/finish and the dispatchers.
+// FP64/FP32, 6 Schwarz, explicit quartets, prep/finish and the dispatchers,
+// 7 one-electron integrals and their gradient.
 #ifndef HFX_PART
-#error "compile with -DHFX_PART=1..6"
+#error "compile with -DHFX_PART=1..7"
 #endif
 
 namespace {
@@ -940,3 +943,191 @@ hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* s
 }
 
 #endif  // HFX_PART == 6
+
+#if HFX_PART == 7
+namespace {
+
+// One wavefront per shell pair (grid-stride over the pair list). Overlap and
+// kinetic terms: lanes over the primitive pairs; nuclear attraction: lanes over
+// the nuclei, primitive pairs in a wavefront-uniform loop.
+template <int LA, int LB, bool DERIV>
+__global__ __launch_bounds__(256) void one_e_kernel(OneEArgs a) {
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    extern __shared__ double gl[];   // 3 nat (gradient)
+    if constexpr (DERIV) {
+        for (int k = threadIdx.x; k < 3 * a.nat; k += blockDim.x) gl[k] = 0.0;
+        __syncthreads();
+    }
+    const int lane = threadIdx.x % 64;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int nwave = gridDim.x * blockDim.x / 64;
+    const RysTables tab = dev_tables();
+    for (int pi = wave; pi < a.npairs; pi += nwave) {
+        const int A = a.pairA[pi], B = a.pairB[pi];
+        const int ata = a.sh_atom[A], atb = a.sh_atom[B];
+        const double* ra = a.xyz + 3 * ata;
+        const double* rb = a.xyz + 3 * atb;
+        const double AB[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+        const double r2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+        const int na = a.sh_nprim[A], nb = a.sh_nprim[B], pa0 = a.sh_prim0[A], pb0 = a.sh_prim0[B];
+        const int fa = a.sh_f0[A], fb = a.sh_f0[B];
+        const double poly = pow(1.0 + sqrt(r2), LA + LB + 1);
+        // primitive pair k -> geometry; false when all its terms are below the screen
+        auto prim = [&](int k, OePrim& q) -> bool {
+            const int ia = k / nb, ib = k - ia * nb;
+            q.a = a.alpha[pa0 + ia];
+            q.b = a.alpha[pb0 + ib];
+            q.p = q.a + q.b;
+            q.oo2p = 0.5 / q.p;
+            const double mu_r2 = q.a * q.b / q.p * r2;
+            if (mu_r2 > 745.0) return false;
+            q.K = a.coef[pa0 + ia] * a.coef[pb0 + ib] * exp(-mu_r2);
+            const double bound = fabs(q.K) * fmax(HFX_PI / q.p * sqrt(HFX_PI / q.p), 2.0 * HFX_PI / q.p * a.ztot) *
+                                 (1.0 + q.p) * (1.0 + q.p) * poly;
+            if (bound < HFX_ONE_E_SCREEN) return false;
+            const double ip = 1.0 / q.p;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                q.P[d] = (q.a * ra[d] + q.b * rb[d]) * ip;
+                q.PA[d] = q.P[d] - ra[d];
+                q.PB[d] = q.P[d] - rb[d];
+            }
+            return true;
+        };
+        if constexpr (DERIV) {
+            double pw[NA][NB], ww[NA][NB];
+#pragma unroll
+            for (int i = 0; i < NA; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const size_t ij = (size_t)(fa + i) + (size_t)(fb + j) * a.n;
+                    const double sc = a.scale[fa + i] * a.scale[fb + j];
+                    pw[i][j] = a.P[ij] * sc;
+                    ww[i][j] = a.W[ij] * sc;
+                }
+            double gA[3] = {0.0, 0.0, 0.0};
+            for (int k = lane; k < na * nb; k += 64) {
+                OePrim q;
+                if (prim(k, q)) oe_st_grad<LA, LB>(q, pw, ww, gA);
+            }
+            for (int c0 = 0; c0 < a.nat; c0 += 64) {
+                const int c = c0 + lane;
+                const bool on = c < a.nat;
+                const double Cc[3] = {on ? a.xyz[3 * c] : 0.0, on ? a.xyz[3 * c + 1] : 0.0,
+                                      on ? a.xyz[3 * c + 2] : 0.0};
+                const double Zc = on ? a.Z[c] : 0.0;
+                double h[3] = {0.0, 0.0, 0.0};
+                for (int k = 0; k < na * nb; k++) {
+                    OePrim q;
+                    if (!prim(k, q)) continue;   // wavefront-uniform
+                    if (on) oe_v_grad<LA, LB>(q, Cc, Zc, AB, tab, pw, h);
+                }
+                if (on) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        // derivative on the centre of A (the other half is the (B, A)
+                        // visit's) and the Hellmann-Feynman term of nucleus c
+                        gA[d] += h[d];
+                        if (h[d] != 0.0) atomicAdd(&gl[3 * c + d], -2.0 * h[d]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = wave_sum(2.0 * gA[d]);
+                if (lane == 0 && v != 0.0) atomicAdd(&gl[3 * ata + d], v);
+            }
+        } else {
+            double S[NA][NB], T[NA][NB];
+#pragma unroll
+            for (int i = 0; i < NA; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) S[i][j] = T[i][j] = 0.0;
+            for (int k = lane; k < na * nb; k += 64) {
+                OePrim q;
+                if (prim(k, q)) oe_st<LA, LB>(q, S, T);
+            }
+#pragma unroll
+            for (int i = 0; i < NA; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const double s_ = wave_sum(S[i][j]), t_ = wave_sum(T[i][j]);
+                    if (lane == 0) {
+                        const int fi = fa + i, fj = fb + j;
+                        const double sc = a.scale[fi] * a.scale[fj];
+                        const size_t ij = (size_t)fi + (size_t)fj * a.n, ji = (size_t)fj + (size_t)fi * a.n;
+                        if (a.S) a.S[ij] = a.S[ji] = sc * s_;
+                        if (a.T) a.T[ij] = a.T[ji] = sc * t_;
+                    }
+                }
+            double V[NA][NB];
+#pragma unroll
+            for (int i = 0; i < NA; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) V[i][j] = 0.0;
+            for (int c0 = 0; c0 < a.nat; c0 += 64) {
+                const int c = c0 + lane;
+                const bool on = c < a.nat;
+                const double Cc[3] = {on ? a.xyz[3 * c] : 0.0, on ? a.xyz[3 * c + 1] : 0.0,
+                                      on ? a.xyz[3 * c + 2] : 0.0};
+                const double Zc = on ? a.Z[c] : 0.0;
+                for (int k = 0; k < na * nb; k++) {
+                    OePrim q;
+                    if (!prim(k, q)) continue;   // wavefront-uniform
+                    if (on) oe_v<LA, LB>(q, Cc, Zc, AB, tab, V);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NA; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const double v_ = wave_sum(V[i][j]);
+                    if (lane == 0 && a.V) {
+                        const int fi = fa + i, fj = fb + j;
+                        const double sc = a.scale[fi] * a.scale[fj];
+                        a.V[(size_t)fi + (size_t)fj * a.n] = a.V[(size_t)fj + (size_t)fi * a.n] = sc * v_;
+                    }
+                }
+        }
+    }
+    if constexpr (DERIV) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < 3 * a.nat; k += blockDim.x)
+            if (gl[k] != 0.0) unsafeAtomicAdd(&a.grad[k], gl[k]);
+    }
+}
+
+template <int LA, int LB>
+hipError_t one_e_l(bool deriv, const OneEArgs& a, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    unsigned nb = (unsigned)((a.npairs + 3) / 4);   // four wavefronts per block
+    if (nb > 2048) nb = 2048;
+    if (deriv) {
+        const size_t lds = 3 * (size_t)a.nat * sizeof(double);
+        if (lds > HFX_ROW_LDS_MAX) return hipErrorInvalidValue;
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute((const void*)one_e_kernel<LA, LB, true>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, HFX_ROW_LDS_MAX);
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
+        hipLaunchKernelGGL((one_e_kernel<LA, LB, true>), dim3(nb), dim3(256), lds, s, a);
+    } else {
+        hipLaunchKernelGGL((one_e_kernel<LA, LB, false>), dim3(nb), dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+typedef hipError_t (*one_e_fn)(bool, const OneEArgs&, hipStream_t);
+const one_e_fn ONE_E_TABLE[3][3] = {{one_e_l<0, 0>, one_e_l<0, 1>, one_e_l<0, 2>},
+                                    {one_e_l<1, 0>, one_e_l<1, 1>, one_e_l<1, 2>},
+                                    {one_e_l<2, 0>, one_e_l<2, 1>, one_e_l<2, 2>}};
+
+}  // namespace
+
+hipError_t hfx_launch_one_e(int la, int lb, bool deriv, const OneEArgs& a, hipStream_t s) {
+    if (la < 0 || la > 2 || lb < 0 || lb > 2) return hipErrorInvalidValue;
+    return ONE_E_TABLE[la][lb](deriv, a, s);
+}
+#endif  // HFX_PART == 7
