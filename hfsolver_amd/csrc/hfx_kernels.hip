@@ -38,10 +38,29 @@ __device__ HFX_INLINE RysTables dev_tables(const double* boys = hfx_dev::BOYS_TA
                      hfx_dev::RYS_CHEBF};
 }
 
+// v of lane i - k within its row of 16 lanes (CTRL = row_shr:k), or of the
+// broadcast lane of the previous row(s) (row_bcast:15 / :31); 0 where there is
+// no source or the row is masked off. On the two 32-bit halves.
+template <int CTRL, int ROW_MASK>
+__device__ HFX_INLINE double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the wavefront, returned to every lane: inclusive prefix sums by DPP
+// row shifts (1, 2, 4, 8) and row broadcasts (lane 15 -> next row, lane 31 ->
+// rows 2, 3), all in registers; lane 63 holds the total.
 __device__ HFX_INLINE double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_move<0x111, 0xf>(v);   // row_shr:1
+    v += dpp_move<0x112, 0xf>(v);   // row_shr:2
+    v += dpp_move<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_move<0x118, 0xf>(v);   // row_shr:8
+    v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 #if HFX_PART == 1
