@@ -10,20 +10,42 @@
 // so the x factor of d/dA_x (ab|cd) is 2 a I_x(i+1, ...) - i I_x(i-1, ...) with
 // I_x the Rys 2-D integrals of eri_rys.h built one unit higher on the
 // differentiated centre. The D derivative follows from translational
-// invariance: d/dD = -(d/dA + d/dB + d/dC). The three centres are processed one
-// after the other for each root so that only one extended table is live.
+// invariance: d/dD = -(d/dA + d/dB + d/dC). For each root the bra centres A and
+// B share one table built one power higher on the bra (a up to l_A + 1 and b
+// up to l_B + 1 by the transfer), then C has its own; only one extended table
+// is live at a time.
 #pragma once
 
 #include "eri_rys.h"
 
-// Extents of the tables of the pass for centre CEN (0: A, 1: B, 2: C).
+// Split policy of the gradient kernels: slice over a (then over c) while a
+// thread would hold more than HFX_GRAD_SPLIT density factors gam. The
+// derivative tables need more registers than the energy's (limit 81); a lower
+// limit removes most register spills but repeats the primitive set-up in every
+// slice -- 54 was the fastest of 81 / 54 / 36 on 64 waters.
+#ifndef HFX_GRAD_SPLIT
+#define HFX_GRAD_SPLIT 54
+#endif
+HFX_HD constexpr bool gsplit_a(int la, int lb, int lc, int ld) {
+    return ncart(la) > 1 && ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > HFX_GRAD_SPLIT;
+}
+HFX_HD constexpr bool gsplit_c(int la, int lb, int lc, int ld) {
+    return gsplit_a(la, lb, lc, ld) && ncart(lc) > 1 && ncart(lb) * ncart(lc) * ncart(ld) > HFX_GRAD_SPLIT;
+}
+HFX_HD constexpr int gnsplit(int la, int lb, int lc, int ld) {
+    return (gsplit_a(la, lb, lc, ld) ? ncart(la) : 1) * (gsplit_c(la, lb, lc, ld) ? ncart(lc) : 1);
+}
+
+// Extents of the tables of the pass for centre CEN (0: A, 1: B, 2: C, 3: A and
+// B from one table built one power higher on the bra).
 template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, int CEN>
 struct GPass {
-    static constexpr int LAB = LA + LB + (CEN < 2 ? 1 : 0);  // highest power built on A
+    static constexpr bool DA = CEN == 0 || CEN == 3, DB = CEN == 1 || CEN == 3;
+    static constexpr int LAB = LA + LB + (DA || DB ? 1 : 0);  // highest power built on A
     static constexpr int LCD = LC + LD + (CEN == 2 ? 1 : 0); // highest power built on C
-    static constexpr int LBx = LB + (CEN == 1 ? 1 : 0);      // kept powers of B
+    static constexpr int LBx = LB + (DB ? 1 : 0);            // kept powers of B
     // kept powers of A: all 0..LA(+1), or the three around the selected one
-    static constexpr int XA = SPA ? (CEN == 0 ? 3 : 1) : (LA + 1 + (CEN == 0 ? 1 : 0));
+    static constexpr int XA = SPA ? (DA ? 3 : 1) : (LA + 1 + (DA ? 1 : 0));
     static constexpr int XC = SPC ? (CEN == 2 ? 3 : 1) : (LC + 1 + (CEN == 2 ? 1 : 0));
 };
 
@@ -37,7 +59,7 @@ HFX_HD HFX_INLINE void grad_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R
                                       [GPass<LA, LB, LC, LD, SPA, SPC, CEN>::XC][LD + 1]) {
     using P = GPass<LA, LB, LC, LD, SPA, SPC, CEN>;
     constexpr int NAB = P::LAB, NCD = P::LCD, LBx = P::LBx;
-    constexpr int LAk = LA + (CEN == 0 ? 1 : 0);  // highest kept a power
+    constexpr int LAk = LA + (P::DA ? 1 : 0);  // highest kept a power
     constexpr int LCk = LC + (CEN == 2 ? 1 : 0);
     R G[NAB + 1][NCD + 1];
     G[0][0] = g00;
@@ -72,14 +94,16 @@ HFX_HD HFX_INLINE void grad_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R
         for (int ib = 0; ib <= LBx; ib++)
 #pragma unroll
             for (int m = 0; m <= NCD; m++) {
+                // entries with ia + ib > NAB are never built (and never used)
                 if (SPA) {
-                    const int off = (CEN == 0) ? xa - 1 : 0;
+                    const int off = P::DA ? xa - 1 : 0;
                     R v = R(0);
 #pragma unroll
-                    for (int t = 0; t <= LAk; t++) v = (ea + off == t) ? H[t][ib][m] : v;
+                    for (int t = 0; t <= LAk; t++)
+                        if (t + ib <= NAB) v = (ea + off == t) ? H[t][ib][m] : v;
                     HA[xa][ib][m] = v;
                 } else {
-                    HA[xa][ib][m] = H[xa][ib][m];
+                    HA[xa][ib][m] = xa + ib <= NAB ? H[xa][ib][m] : R(0);
                 }
             }
     R Kt[P::XA][LBx + 1][NCD + 1][LD + 1];
@@ -116,9 +140,11 @@ HFX_HD HFX_INLINE void grad_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R
 }
 
 // Contraction of one pass with gam for one root: adds to g[3*CEN + {0,1,2}].
+// CEN = 3: both bra centres from one table, A into g[0..2] (exponent two_exp)
+// and B into g[3..5] (exponent two_exp_b).
 template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, int CEN, typename R>
 HFX_HD HFX_INLINE void grad_pass(const double* c00, const double* d00, R b10, R b01, R b00, const R* abv,
-                                 const R* cdv, R g00z, const int* sel_a, const int* sel_c, R two_exp,
+                                 const R* cdv, R g00z, const int* sel_a, const int* sel_c, R two_exp, R two_exp_b,
                                  const R (&gam)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT], R* g) {
     using P = GPass<LA, LB, LC, LD, SPA, SPC, CEN>;
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
@@ -132,14 +158,15 @@ HFX_HD HFX_INLINE void grad_pass(const double* c00, const double* d00, R b10, R 
     grad_dir<LA, LB, LC, LD, SPA, SPC, CEN, R>(R(c00[2]), R(d00[2]), b10, b01, b00, abv[2], cdv[2], g00z, sel_a[2],
                                                sel_c[2], Tz);
     R gx = R(0), gy = R(0), gz = R(0);
+    R hx = R(0), hy = R(0), hz = R(0);   // B components of CEN = 3
 #pragma unroll
     for (int a = 0; a < Q::NAt; a++) {
         // exponents of a: table index of power p is p (unsliced) or 1 (+-1) around the selection
         const int ex = SPA ? sel_a[0] : cart_exp(LA, a, 0), ey = SPA ? sel_a[1] : cart_exp(LA, a, 1),
                   ez = SPA ? sel_a[2] : cart_exp(LA, a, 2);
-        const int ax = SPA ? (CEN == 0 ? 1 : 0) : cart_exp(LA, a, 0);
-        const int ay = SPA ? (CEN == 0 ? 1 : 0) : cart_exp(LA, a, 1);
-        const int az = SPA ? (CEN == 0 ? 1 : 0) : cart_exp(LA, a, 2);
+        const int ax = SPA ? (P::DA ? 1 : 0) : cart_exp(LA, a, 0);
+        const int ay = SPA ? (P::DA ? 1 : 0) : cart_exp(LA, a, 1);
+        const int az = SPA ? (P::DA ? 1 : 0) : cart_exp(LA, a, 2);
 #pragma unroll
         for (int b = 0; b < Q::NB; b++) {
             const int bx = cart_exp(LB, b, 0), by = cart_exp(LB, b, 1), bz = cart_exp(LB, b, 2);
@@ -156,7 +183,15 @@ HFX_HD HFX_INLINE void grad_pass(const double* c00, const double* d00, R b10, R 
                     const R gm = gam[((a * Q::NB + b) * Q::NCt + c) * Q::ND + d];
                     const R x0 = Tx[ax][bx][cx][dx], y0 = Ty[ay][by][cy][dy], z0 = Tz[az][bz][cz][dz];
                     R dxv, dyv, dzv;
-                    if (CEN == 0) {
+                    if (CEN == 3) {
+                        const R ux = two_exp_b * Tx[ax][bx + 1][cx][dx] - (bx > 0 ? R(bx) * Tx[ax][bx > 0 ? bx - 1 : 0][cx][dx] : R(0));
+                        const R uy = two_exp_b * Ty[ay][by + 1][cy][dy] - (by > 0 ? R(by) * Ty[ay][by > 0 ? by - 1 : 0][cy][dy] : R(0));
+                        const R uz = two_exp_b * Tz[az][bz + 1][cz][dz] - (bz > 0 ? R(bz) * Tz[az][bz > 0 ? bz - 1 : 0][cz][dz] : R(0));
+                        hx += gm * ux * y0 * z0;
+                        hy += gm * x0 * uy * z0;
+                        hz += gm * x0 * y0 * uz;
+                    }
+                    if (CEN == 0 || CEN == 3) {
                         dxv = two_exp * Tx[ax + 1][bx][cx][dx] - (ex > 0 ? R(ex) * Tx[ax > 0 ? ax - 1 : 0][bx][cx][dx] : R(0));
                         dyv = two_exp * Ty[ay + 1][by][cy][dy] - (ey > 0 ? R(ey) * Ty[ay > 0 ? ay - 1 : 0][by][cy][dy] : R(0));
                         dzv = two_exp * Tz[az + 1][bz][cz][dz] - (ez > 0 ? R(ez) * Tz[az > 0 ? az - 1 : 0][bz][cz][dz] : R(0));
@@ -176,9 +211,15 @@ HFX_HD HFX_INLINE void grad_pass(const double* c00, const double* d00, R b10, R 
             }
         }
     }
-    g[3 * CEN + 0] += gx;
-    g[3 * CEN + 1] += gy;
-    g[3 * CEN + 2] += gz;
+    const int g0 = CEN == 3 ? 0 : 3 * CEN;
+    g[g0 + 0] += gx;
+    g[g0 + 1] += gy;
+    g[g0 + 2] += gz;
+    if (CEN == 3) {
+        g[3] += hx;
+        g[4] += hy;
+        g[5] += hz;
+    }
 }
 
 // g[0..8] += sum_{abcd in slice} gam_abcd d(ab|cd)/d(A, B, C) for the quartet.
@@ -249,12 +290,10 @@ HFX_HD HFX_INLINE void grad_quartet_k(const ShellPair& bra, const PrimPair* __re
                 const double c00[3] = {PA[0] - fp * PQ[0], PA[1] - fp * PQ[1], PA[2] - fp * PQ[2]};
                 const double d00[3] = {QC[0] + fq * PQ[0], QC[1] + fq * PQ[1], QC[2] + fq * PQ[2]};
                 const R wz = R(w[r] * pref);
-                grad_pass<LA, LB, LC, LD, SPA, SPC, 0, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
-                                                          R(2.0 * EB.aa), gam, gl);
-                grad_pass<LA, LB, LC, LD, SPA, SPC, 1, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
-                                                          R(2.0 * EB.ab), gam, gl);
+                grad_pass<LA, LB, LC, LD, SPA, SPC, 3, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
+                                                          R(2.0 * EB.aa), R(2.0 * EB.ab), gam, gl);
                 grad_pass<LA, LB, LC, LD, SPA, SPC, 2, R>(c00, d00, b10, b01, b00, abv, cdv, wz, sel_a, sel_c,
-                                                          two_c, gam, gl);
+                                                          two_c, R(0), gam, gl);
             }
         }
     }

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
 // (= -(A + B + C), translational invariance) by LDS atomics on their atoms.
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void grad_row_kernel(GradRowArgs a) {
-    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    constexpr bool SPA = gsplit_a(LA, LB, LC, LD), SPC = gsplit_c(LA, LB, LC, LD);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
     constexpr int nrow = NAt + NB;
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(256) void grad_row_kernel(GradRowArgs a) {
 // Two-electron gradient: one block = one bra shell pair x a chunk of its kets.
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
-    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    constexpr bool SPA = gsplit_a(LA, LB, LC, LD), SPC = gsplit_c(LA, LB, LC, LD);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NRG = (LA + LB + LC + LD + 1) / 2 + 1;
     extern __shared__ double smem[];
@@ -759,7 +759,7 @@ hipError_t grad_xy(const GradArgs& a, int nblocks, hipStream_t s) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    dim3 grid((unsigned)nblocks, (unsigned)gnsplit(HFX_CLS(X), HFX_CLS(Y)));
     hipLaunchKernelGGL((grad_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }
@@ -768,7 +768,7 @@ const grad_fn GRAD_TABLE[6][6] = HFX_TABLE(grad_xy);
 
 template <int X, int Y>
 hipError_t grad_rows_xy(const GradRowArgs& a, int nblocks, hipStream_t s) {
-    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
+    constexpr int NAt = gsplit_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
     constexpr int NB = ncart(pair_class_lb(X));
     const size_t lds = hfx_grad_row_lds(NAt + NB, a.ncols, a.R, a.dlds, sizeof(GradReal) == 4, a.nat);
     if (lds > HFX_ROW_LDS_MAX || a.R < 1 || a.R > HFX_ROWS_MAX || a.kt.S < 1 || a.kt.S > HFX_SUB_MAX)
@@ -780,7 +780,7 @@ hipError_t grad_rows_xy(const GradRowArgs& a, int nblocks, hipStream_t s) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    dim3 grid((unsigned)nblocks, (unsigned)gnsplit(HFX_CLS(X), HFX_CLS(Y)));
     hipLaunchKernelGGL((grad_row_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }

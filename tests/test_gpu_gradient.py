@@ -99,3 +99,19 @@ def test_mixed_precision_gradient_dimer():
     assert np.abs(g2 - g64).max() < 1e-6
     f64.close()
     f32.close()
+
+
+def test_gradient_fallback_kernel_matches_row_mode():
+    """kernel_mode=1 runs the per-sublist tile / gradient kernels (the fallback
+    where the row-mode LDS does not fit); same two-electron gradient."""
+    atno, xyz = water_cluster(2, seed=2)
+    say("two-electron gradient, row mode vs fallback kernels")
+    r = rhf_gauss(atno, xyz, Nscf=80, tolE=1e-12, tolP=1e-10, diis=True, keep_engine=True)
+    nocc = int(sum(atno)) // 2
+    P = 2.0 * r.C[:, :nocc] @ r.C[:, :nocc].T
+    g_rows = r.engine.grad_2e(P)
+    r.engine.close()
+    eng = Engine(Basis(atno, xyz), kernel_mode=1)
+    g_tiles = eng.grad_2e(P)
+    eng.close()
+    assert np.abs(g_rows - g_tiles).max() < 1e-11

@@ -104,3 +104,41 @@ def test_rys_rule_reproduces_boys_moments(lib):
                 assert np.all(np.diff(u[:nr]) > 0) and np.all(u[:nr] > 0) and np.all(u[:nr] < 1)
                 for k in range(2 * nr):
                     assert abs(np.sum(w[:nr] * u[:nr] ** k) - Ff[k]) <= 2e-7 * Ff[0] + 1e-6 * Ff[k], (nr, T, k)
+
+
+def test_derivative_quartets_match_oracle(lib):
+    """grad_rys.h instantiated on the host: sum gam_abcd d(ab|cd)/d(A, B, C) for
+    every class combination against the oracle's derivative integrals
+    (orc_deri_bf: the same THO restatement differentiated on each centre), in
+    FP64 and FP32 arithmetic."""
+    atno, xyz = water_cluster(2, seed=2)
+    Bas = O.Basis(atno, xyz)
+    sh = KC.shells_of(Bas)
+    rng = np.random.default_rng(5)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    seen = set()
+    for _ in range(40000):
+        if len(seen) == 21:
+            break
+        A, B, C, D = _oriented(*[sh[i] for i in rng.integers(0, len(sh), 4)])
+        key = (A["L"], B["L"], C["L"], D["L"])
+        if key in seen:
+            continue
+        seen.add(key)
+        nc = lambda s: (s["L"] + 1) * (s["L"] + 2) // 2
+        dims = [nc(s) for s in (A, B, C, D)]
+        gam = rng.standard_normal(dims)
+        scale = np.einsum("a,b,c,d->abcd", A["sc"], B["sc"], C["sc"], D["sc"])
+        gs = np.ascontiguousarray(gam * scale)
+        ref = np.zeros((4, 3))
+        for a, b, c, d in itertools.product(*(range(x) for x in dims)):
+            ref += gam[a, b, c, d] * O.deri(Bas, A["f0"] + a, B["f0"] + b, C["f0"] + c, D["f0"] + d)
+        m = np.abs(ref).max()
+        for fp32, tol in ((0, 1e-12), (1, 2e-5)):
+            g = np.zeros(9)
+            args = [A["L"], B["L"], C["L"], D["L"]] + [p(s["cen"]) for s in (A, B, C, D)]
+            for s in (A, B, C, D):
+                args += [len(s["a"]), p(s["a"]), p(s["c"])]
+            assert lib.chk_grad_quartet(*args, p(gs), fp32, p(g)) == 0
+            assert np.abs(g.reshape(3, 3) - ref[:3]).max() <= 1e-14 + tol * m, (key, fp32)
+    assert len(seen) == 21
