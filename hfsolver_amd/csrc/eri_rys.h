@@ -187,9 +187,10 @@ struct KetPrimsAoSG : KetPrimsAoS {
 // bra: shell pair with l_A = LA, l_B = LB (primitive pairs prims[bra.p0 ..]);
 // ket: l_C = LC, l_D = LD, C - D = (cdx, cdy, cdz), primitives from kp.
 // Ket primitives form the outer loop (one per-lane load each); the bra's are
-// the same for the whole wavefront.
-template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R, class KP>
-HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const PrimPair* __restrict__ prims, double cdx_,
+// the same for the whole wavefront, read as PrimPair or as its single-precision
+// copy PrimPairF (BP).
+template <int LA, int LB, int LC, int LD, bool SPA, bool SPC, typename R, class KP, class BP>
+HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const BP* __restrict__ prims, double cdx_,
                                      double cdy_, double cdz_, const KP& kp, int sa, int sc, const RysTables& tab,
                                      R (&eri)[QClass<LA, LB, LC, LD, SPA, SPC>::NOUT]) {
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
@@ -223,7 +224,7 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const PrimPair* __res
     for (int k1 = 0; k1 < nkp; k1++) {
         const auto Qp = kp.get(k1);
         for (int i1 = 0; i1 < bra.np; i1++) {
-            const PrimPair P = hfx_ld_prim(prims + bra.p0 + i1);
+            const BP P = hfx_ld_prim(prims + bra.p0 + i1);
             // primitive-level arithmetic in the kernel's type (FP32 kernels: roots
             // and weights from the single-precision rules and tables)
             const PT p = PT(P.p), q = PT(Qp.p), pq = p + q;

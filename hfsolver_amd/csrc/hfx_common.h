@@ -61,12 +61,19 @@ struct PrimPair {
     double pad;
 };
 
+// Single-precision copy of a primitive pair (same fields), read by the FP32
+// kernels so that their primitive loops take the bra values without conversion.
+struct PrimPairF {
+    float p, ip, Px, Py, Pz, PAx, PAy, PAz, K, pad;
+};
+
 // Exponents of a primitive pair (used by the derivative integrals).
 struct PrimExp {
     double aa, ab;
 };
 
-static_assert(sizeof(ShellPair) == 80 && sizeof(PrimPair) == 80, "record layouts read field by field below");
+static_assert(sizeof(ShellPair) == 80 && sizeof(PrimPair) == 80 && sizeof(PrimPairF) == 40,
+              "record layouts read field by field below");
 
 // Loads of read-only data whose address is the same for the whole wavefront:
 // on the device through the constant address space, so that the compiler can
@@ -102,6 +109,36 @@ HFX_HD HFX_INLINE PrimPair hfx_ld_prim(const PrimPair* p) {
     return *p;
 #endif
 }
+
+HFX_HD HFX_INLINE PrimPairF hfx_ld_prim(const PrimPairF* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const HFX_AS4 float* d = (const HFX_AS4 float*)p;
+    PrimPairF r;
+    r.p = d[0];
+    r.ip = d[1];
+    r.Px = d[2];
+    r.Py = d[3];
+    r.Pz = d[4];
+    r.PAx = d[5];
+    r.PAy = d[6];
+    r.PAz = d[7];
+    r.K = d[8];
+    r.pad = 0.0f;
+    return r;
+#else
+    return *p;
+#endif
+}
+
+// Primitive-pair record a kernel of arithmetic type R reads for its bra.
+template <typename R>
+struct BraPrimOf {
+    using type = PrimPair;
+};
+template <>
+struct BraPrimOf<float> {
+    using type = PrimPairF;
+};
 
 HFX_HD HFX_INLINE PrimExp hfx_ld_pexp(const PrimExp* p) {
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -124,6 +124,7 @@ struct hfx_engine {
     // device memory
     ShellPair* d_pairs = nullptr;
     PrimPair* d_prims = nullptr;
+    PrimPairF* d_primsf = nullptr;
     PrimExp* d_pexp = nullptr;
     int* d_cls_list[HFX_NPAIRCLASS] = {nullptr};
     double* d_scale = nullptr;
@@ -192,6 +193,7 @@ void free_geometry(hfx_t* h) {
     for (int c = 0; c < HFX_NPAIRCLASS; c++) h->ktab[c] = hfx_engine::KetTable{};
     h->d_pairs = nullptr;
     h->d_prims = nullptr;
+    h->d_primsf = nullptr;
     h->d_pexp = nullptr;
     for (int c = 0; c < HFX_NPAIRCLASS; c++) {
         h->d_cls_list[c] = nullptr;
@@ -696,6 +698,30 @@ int build_lists(hfx_t* h) {
                         bk.push_back(k0);
                     }
                 }
+                if (h->opts.verbose) {
+                    // lane use of the launch: primitive quartets over the lane slots of the
+                    // wavefronts that hold at least one ket
+                    double used = 0, slots = 0;
+                    for (size_t b = 0; b < br.size(); b++)
+                        for (int s = 0; s < S; s++) {
+                            int lo = 1 << 30, hi = 0;
+                            for (int i = br[b]; i < std::min(nb, br[b] + R); i++) {
+                                const size_t x = (size_t)i * S + s;
+                                if (rke[prec][x] > rkb[prec][x]) {
+                                    lo = std::min(lo, rkb[prec][x]);
+                                    hi = std::max(hi, rke[prec][x]);
+                                }
+                            }
+                            const int j0 = std::max(bk[b], lo), j1 = std::min(bk[b] + chunk, hi);
+                            if (j1 <= j0) continue;
+                            const double w = (double)h->ktab[ck].t.np[s] * h->pairs[bl[br[b]]].np;
+                            used += w * (j1 - j0);
+                            slots += w * ((j1 - j0 + 63) / 64) * 64;
+                        }
+                    fprintf(stderr, "hfx:   rows (%d%d|%d%d) %s: lane use %.3f of %.4g primitive-quartet slots\n",
+                            pair_class_la(cb), pair_class_lb(cb), lc, ld, prec ? "fp32" : "fp64",
+                            slots > 0 ? used / slots : 0.0, slots);
+                }
                 rc.chunk[prec] = chunk;
                 rc.nblk[prec] = (int)br.size();
                 if (upload(h, &rc.d_kbeg[prec], rkb[prec], true)) return 1;
@@ -731,6 +757,15 @@ int setup_geometry(hfx_t* h) {
     build_pairs(h);
     if (upload(h, &h->d_pairs, h->pairs, true)) return 1;
     if (upload(h, &h->d_prims, h->prims, true)) return 1;
+    {
+        std::vector<PrimPairF> pf(h->prims.size());
+        for (size_t i = 0; i < pf.size(); i++) {
+            const PrimPair& q = h->prims[i];
+            pf[i] = PrimPairF{(float)q.p, (float)q.ip, (float)q.Px, (float)q.Py, (float)q.Pz,
+                              (float)q.PAx, (float)q.PAy, (float)q.PAz, (float)q.K, 0.0f};
+        }
+        if (upload(h, &h->d_primsf, pf, true)) return 1;
+    }
     if (upload(h, &h->d_pexp, h->pexp, true)) return 1;
     if (compute_schwarz(h)) return 1;
     HIPCHK(h, hipMemcpy(h->d_pairs, h->pairs.data(), h->pairs.size() * sizeof(ShellPair), hipMemcpyHostToDevice));
@@ -895,6 +930,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             RowArgs r;
             r.pairs = h->d_pairs;
             r.prims = h->d_prims;
+            r.primsf = h->d_primsf;
             r.bra_list = h->d_cls_list[c.cb];
             r.kt = h->ktab[c.ck].t;
             r.kbeg = c.d_kbeg[prec];

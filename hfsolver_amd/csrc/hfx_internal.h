@@ -74,6 +74,7 @@ struct KetTab {
 struct RowArgs {
     const ShellPair* pairs;
     const PrimPair* prims;
+    const PrimPairF* primsf;   // single-precision copy of prims (FP32 launches)
     const int* bra_list;
     KetTab kt;
     const int* kbeg;

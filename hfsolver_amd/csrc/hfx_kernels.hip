@@ -177,6 +177,27 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
     const int nk = kt.nk;
     const int k0 = hfx_ld_int(a.blk_k0 + blockIdx.x), k1 = k0 + a.chunk;
     const bool lead = (threadIdx.x % 64) == 0;
+    using BP = typename BraPrimOf<R>::type;
+    const BP* bprims;
+    if constexpr (sizeof(R) == 4)
+        bprims = a.primsf;
+    else
+        bprims = a.prims;
+    // one-row blocks (the usual case): the bra's D_ab loaded once, its J_ab summed
+    // in registers over all kets of the block and reduced over the wavefront at the end
+    const bool onerow = nr == 1;
+    R dab0[NAt][NB];
+    double jab[NAt][NB];
+    {
+        const ShellPair bra0 = hfx_ld_pair(a.pairs + hfx_ld_int(a.bra_list + row0));
+#pragma unroll
+        for (int i = 0; i < NAt; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                dab0[i][j] = R(a.D[(size_t)(bra0.fa + sa + i) * n + bra0.fb + j]);
+                jab[i][j] = 0.0;
+            }
+    }
     for (int s = 0; s < kt.S; s++) {
         // ket range of the block in sublist s: union of its rows' ranges within the chunk
         int lo = 1 << 30, hi = 0;
@@ -232,13 +253,20 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
                     sf = bra.A == bra.B ? 0.5 * sket : sket;
                     if (bp == ki.pid) sf *= 0.5;
                     R eri[Q::NOUT];
-                    eri_quartet_k<LA, LB, LC, LD, SPA, SPC, R>(bra, a.prims, cdx, cdy, cdz, kp, sa, sc,
+                    eri_quartet_k<LA, LB, LC, LD, SPA, SPC, R>(bra, bprims, cdx, cdy, cdz, kp, sa, sc,
                                                                dev_tables(), eri);
                     JKDens<LA, LB, LC, LD, SPA, SPC, R> dn;
+                    if (onerow) {
 #pragma unroll
-                    for (int i = 0; i < NAt; i++)
+                        for (int i = 0; i < NAt; i++)
 #pragma unroll
-                        for (int j = 0; j < NB; j++) dn.ab[i][j] = R(a.D[(size_t)(fa + i) * n + fb + j]);
+                            for (int j = 0; j < NB; j++) dn.ab[i][j] = dab0[i][j];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < NAt; i++)
+#pragma unroll
+                            for (int j = 0; j < NB; j++) dn.ab[i][j] = R(a.D[(size_t)(fa + i) * n + fb + j]);
+                    }
 #pragma unroll
                     for (int c = 0; c < NCt; c++)
 #pragma unroll
@@ -300,14 +328,21 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
                             atomicAdd(&Kr[(NAt + b) * ncols + cd + d], sf * double(acc.Kbd[b][d]));
                     }
                 }
-                // J_ab of row r: reduce over the wavefront, one LDS add per wavefront
+                if (onerow) {
 #pragma unroll
-                for (int i = 0; i < NAt; i++)
+                    for (int i = 0; i < NAt; i++)
 #pragma unroll
-                    for (int j = 0; j < NB; j++) {
-                        const double v = wave_sum(2.0 * sf * double(acc.Jab[i][j]));
-                        if (lead && v != 0.0) atomicAdd(&Jl[(r * NAt + i) * NB + j], v);
-                    }
+                        for (int j = 0; j < NB; j++) jab[i][j] += 2.0 * sf * double(acc.Jab[i][j]);
+                } else {
+                    // J_ab of row r: reduce over the wavefront, one LDS add per wavefront
+#pragma unroll
+                    for (int i = 0; i < NAt; i++)
+#pragma unroll
+                        for (int j = 0; j < NB; j++) {
+                            const double v = wave_sum(2.0 * sf * double(acc.Jab[i][j]));
+                            if (lead && v != 0.0) atomicAdd(&Jl[(r * NAt + i) * NB + j], v);
+                        }
+                }
             }
             if (incol) {
 #pragma unroll
@@ -319,6 +354,15 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
                     }
             }
         }
+    }
+    if (onerow) {
+#pragma unroll
+        for (int i = 0; i < NAt; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                const double v = wave_sum(jab[i][j]);
+                if (lead && v != 0.0) atomicAdd(&Jl[i * NB + j], v);
+            }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < nr * nrow * ncols; k += blockDim.x) {

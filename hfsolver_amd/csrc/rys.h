@@ -29,9 +29,11 @@ HFX_HD HFX_INLINE double hfx_rsqrt(double x) {
 #endif
 }
 
+// Single precision: the hardware reciprocal square root (1 ulp) on the device;
+// every caller passes a positive normal number (p + q, or T beyond the table).
 HFX_HD HFX_INLINE float hfx_rsqrt(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return ::rsqrtf(x);
+    return __builtin_amdgcn_rsqf(x);
 #else
     return 1.0f / std::sqrt(x);
 #endif
