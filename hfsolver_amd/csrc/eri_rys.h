@@ -223,18 +223,20 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const BP* __restrict_
     const int nkp = kp.count();
     for (int k1 = 0; k1 < nkp; k1++) {
         const auto Qp = kp.get(k1);
+        const PT q = PT(Qp.p), iq = PT(Qp.ip);
+        const PT kq = PT(Qp.K) * iq;   // the ket's factor K / q of the prefactor
         for (int i1 = 0; i1 < bra.np; i1++) {
             const BP P = hfx_ld_prim(prims + bra.p0 + i1);
             // primitive-level arithmetic in the kernel's type (FP32 kernels: roots
             // and weights from the single-precision rules and tables)
-            const PT p = PT(P.p), q = PT(Qp.p), pq = p + q;
-            const PT ip = PT(P.ip), iq = PT(Qp.ip);
+            const PT p = PT(P.p), pq = p + q;
+            const PT ip = PT(P.ip);
             const PT isq = hfx_rsqrt(pq);
             const PT ipq = isq * isq;
             const PT rho = p * q * ipq;
             const PT PQx = PT(P.Px - Qp.Px), PQy = PT(P.Py - Qp.Py), PQz = PT(P.Pz - Qp.Pz);
             const PT T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const PT pref = PT(HFX_TWO_PI_52) * PT(P.K) * PT(Qp.K) * ip * iq * isq;
+            const PT pref = PT(P.Kp) * kq * isq;
             PT u[NR], w[NR];
             if constexpr (NR <= 2) {
                 if constexpr (sizeof(PT) == 4)

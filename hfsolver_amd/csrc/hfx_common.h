@@ -58,13 +58,13 @@ struct PrimPair {
     double Px, Py, Pz;    // (alpha_a A + alpha_b B) / p
     double PAx, PAy, PAz; // P - A
     double K;             // c_a c_b exp(-alpha_a alpha_b |A-B|^2 / p), shell-level coefficients
-    double pad;
+    double Kp;            // 2 pi^{5/2} K / p: the bra's factor of the ERI prefactor
 };
 
 // Single-precision copy of a primitive pair (same fields), read by the FP32
 // kernels so that their primitive loops take the bra values without conversion.
 struct PrimPairF {
-    float p, ip, Px, Py, Pz, PAx, PAy, PAz, K, pad;
+    float p, ip, Px, Py, Pz, PAx, PAy, PAz, K, Kp;
 };
 
 // Exponents of a primitive pair (used by the derivative integrals).
@@ -103,7 +103,7 @@ HFX_HD HFX_INLINE PrimPair hfx_ld_prim(const PrimPair* p) {
     r.PAy = d[6];
     r.PAz = d[7];
     r.K = d[8];
-    r.pad = 0.0;
+    r.Kp = d[9];
     return r;
 #else
     return *p;
@@ -123,7 +123,7 @@ HFX_HD HFX_INLINE PrimPairF hfx_ld_prim(const PrimPairF* p) {
     r.PAy = d[6];
     r.PAz = d[7];
     r.K = d[8];
-    r.pad = 0.0f;
+    r.Kp = d[9];
     return r;
 #else
     return *p;

@@ -337,7 +337,6 @@ void build_pairs(hfx_t* h) {
                     PrimPair pp;
                     pp.p = p;
                     pp.ip = 1.0 / p;
-                    pp.pad = 0.0;
                     pp.Px = (a * ra[0] + b * rb[0]) / p;
                     pp.Py = (a * ra[1] + b * rb[1]) / p;
                     pp.Pz = (a * ra[2] + b * rb[2]) / p;
@@ -345,6 +344,7 @@ void build_pairs(hfx_t* h) {
                     pp.PAy = pp.Py - ra[1];
                     pp.PAz = pp.Pz - ra[2];
                     pp.K = h->coef[sa.prim0 + pa] * h->coef[sb.prim0 + pb] * std::exp(-a * b / p * r2);
+                    pp.Kp = HFX_TWO_PI_52 * pp.K / p;
                     if (prim_bound(pp.K, p) < qcut) continue;
                     h->prims.push_back(pp);
                     h->pexp.push_back(PrimExp{a, b});
@@ -762,7 +762,7 @@ int setup_geometry(hfx_t* h) {
         for (size_t i = 0; i < pf.size(); i++) {
             const PrimPair& q = h->prims[i];
             pf[i] = PrimPairF{(float)q.p, (float)q.ip, (float)q.Px, (float)q.Py, (float)q.Pz,
-                              (float)q.PAx, (float)q.PAy, (float)q.PAz, (float)q.K, 0.0f};
+                              (float)q.PAx, (float)q.PAy, (float)q.PAz, (float)q.K, (float)q.Kp};
         }
         if (upload(h, &h->d_primsf, pf, true)) return 1;
     }

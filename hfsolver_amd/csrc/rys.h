@@ -39,6 +39,16 @@ HFX_HD HFX_INLINE float hfx_rsqrt(float x) {
 #endif
 }
 
+// Square root in single precision: the hardware instruction (1 ulp) on the
+// device; the argument (a discriminant of two distinct real roots) is positive.
+HFX_HD HFX_INLINE float hfx_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return std::sqrt(x);
+#endif
+}
+
 // 1/x in single precision: the hardware reciprocal (1 ulp) on the device.
 HFX_HD HFX_INLINE float hfx_rcp(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -171,7 +181,7 @@ HFX_HD HFX_INLINE void rys_roots_small_f(float T, float* u, float* w, const floa
         const float id = hfx_rcp(m0 * m2 - m1 * m1);
         const float a = (m1 * m2 - m0 * m3) * id;
         const float b = (m1 * m3 - m2 * m2) * id;
-        const float u2 = 0.5f * (-a + std::sqrt(a * a - 4.0f * b));
+        const float u2 = 0.5f * (-a + hfx_sqrt(a * a - 4.0f * b));
         const float u1 = b * hfx_rcp(u2);
         const float w1 = (m1 - u2 * m0) * hfx_rcp(u1 - u2);
         u[0] = u1;

@@ -40,7 +40,6 @@ void make_pair(int la, int lb, const double* A, const double* B, int na, const d
             PrimPair pp;
             pp.p = p;
             pp.ip = 1.0 / p;
-            pp.pad = 0.0;
             pp.Px = (a * A[0] + b * B[0]) / p;
             pp.Py = (a * A[1] + b * B[1]) / p;
             pp.Pz = (a * A[2] + b * B[2]) / p;
@@ -48,6 +47,7 @@ void make_pair(int la, int lb, const double* A, const double* B, int na, const d
             pp.PAy = pp.Py - A[1];
             pp.PAz = pp.Pz - A[2];
             pp.K = ca[i] * cb[j] * std::exp(-a * b / p * r2);
+            pp.Kp = HFX_TWO_PI_52 * pp.K / p;
             prims.push_back(pp);
             if (pexp) pexp->push_back(PrimExp{a, b});
         }
