@@ -110,6 +110,7 @@ struct hfx_engine {
         int R = 1, dlds = 0, ncols = 0, offd = 0;
         int nblk[2] = {0, 0};
         int chunk[2] = {0, 0};
+        double cost[2] = {0, 0};   // modelled flops of the launch (orders the launches)
         int* d_kbeg[2] = {nullptr, nullptr};
         int* d_kend[2] = {nullptr, nullptr};
         int* d_blk_row[2] = {nullptr, nullptr};
@@ -149,6 +150,11 @@ struct hfx_engine {
     double* d_xyz = nullptr;   // 3 nat, geometry
     double* d_grad = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // extra streams (default priority) over which the class launches of a Fock
+    // build are dealt, so that the tail of one launch overlaps the next (HFX_STREAMS)
+    int nstreams = 3;
+    hipStream_t aux[HFX_AUX_STREAMS] = {};
+    hipEvent_t evaux[HFX_AUX_STREAMS + 1] = {};
     std::vector<void*> geom_allocs;   // freed on geometry change
 };
 
@@ -649,6 +655,13 @@ int build_lists(hfx_t* h) {
             if (!rowmode) continue;
             rc.R = R;
             rc.dlds = dlds;
+            {
+                const FlopModel fm = flop_model(pair_class_la(cb), pair_class_lb(cb), lc, ld);
+                const double nint = (double)ncart(pair_class_la(cb)) * ncart(pair_class_lb(cb)) * ncart(lc) * ncart(ld);
+                for (size_t ci = first_combo; ci < h->combos.size(); ci++)
+                    for (int prec = 0; prec < 2; prec++)
+                        rc.cost[prec] += h->combos[ci].nprimq[prec] * fm.per_prim + h->combos[ci].nq[prec] * nint * fm.per_int;
+            }
             // blocks: (first row, first ket position) for groups of R rows, chunks of the
             // table positions that meet at least one of the group's ranges
             for (int prec = 0; prec < 2; prec++) {
@@ -875,6 +888,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (const char* e = std::getenv("HFX_ROW_RMAX")) h->row_rmax = std::max(1, std::min(HFX_ROWS_MAX, std::atoi(e)));
     if (const char* e = std::getenv("HFX_ROW_PAIR_KB")) h->row_pair_bytes = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
+    if (const char* e = std::getenv("HFX_STREAMS")) h->nstreams = std::max(1, std::min(HFX_AUX_STREAMS + 1, std::atoi(e)));
     h->nat = nat;
     h->atno.assign(atno, atno + nat);
     h->ztot = 0;
@@ -889,6 +903,8 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
+    for (int i = 0; i + 1 < h->nstreams; i++) HIPCHK(h, hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking));
+    for (int i = 0; i < h->nstreams; i++) HIPCHK(h, hipEventCreateWithFlags(&h->evaux[i], hipEventDisableTiming));
     const size_t nn = (size_t)nbf * nbf;
     if (upload(h, &h->d_scale, h->scale, false)) return 1;
     for (int L = 0; L <= HFX_LMAX; L++)
@@ -924,9 +940,32 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
         if (h->ktab[c].t.nk > 0)
             HIPCHK(h, hfx_launch_ket_gather(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_D, n, h->stream));
-    for (const auto& c : h->rcombos) {
-        for (int prec = 0; prec < 2; prec++) {
-            if (c.nblk[prec] == 0) continue;
+    // class launches dealt round-robin over the main and the extra streams,
+    // which start after the gathers and are joined before the scatters
+    const int ns = h->nstreams;
+    if (ns > 1) {
+        HIPCHK(h, hipEventRecord(h->evaux[0], h->stream));
+        for (int i = 0; i + 1 < ns; i++) HIPCHK(h, hipStreamWaitEvent(h->aux[i], h->evaux[0], 0));
+    }
+    // largest launches first, each to the stream with the least modelled work so far,
+    // so that the small launches at the end fill the tails of the large ones
+    std::vector<std::pair<double, int>> order;
+    for (size_t i = 0; i < h->rcombos.size(); i++)
+        for (int prec = 0; prec < 2; prec++)
+            if (h->rcombos[i].nblk[prec] > 0) order.push_back({h->rcombos[i].cost[prec], (int)(2 * i + prec)});
+    std::stable_sort(order.begin(), order.end(), [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+        return x.first > y.first;
+    });
+    double load[HFX_AUX_STREAMS + 1] = {};
+    for (const auto& o : order) {
+        const auto& c = h->rcombos[o.second / 2];
+        const int prec = o.second % 2;
+        {
+            int si = 0;
+            for (int i = 1; i < ns; i++)
+                if (load[i] < load[si]) si = i;
+            load[si] += o.first;
+            hipStream_t s = si == 0 ? h->stream : h->aux[si - 1];
             RowArgs r;
             r.pairs = h->d_pairs;
             r.prims = h->d_prims;
@@ -949,8 +988,12 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.J = h->d_Jb;
             r.K = h->d_Kb;
             r.n = n;
-            HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], h->stream));
+            HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], s));
         }
+    }
+    for (int i = 0; i + 1 < ns; i++) {
+        HIPCHK(h, hipEventRecord(h->evaux[i + 1], h->aux[i]));
+        HIPCHK(h, hipStreamWaitEvent(h->stream, h->evaux[i + 1], 0));
     }
     JKArgs a;
     a.pairs = h->d_pairs;
@@ -1378,6 +1421,10 @@ void hfx_destroy(hfx_t* h) {
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    for (int i = 0; i <= HFX_AUX_STREAMS; i++)
+        if (h->evaux[i]) (void)hipEventDestroy(h->evaux[i]);
+    for (int i = 0; i < HFX_AUX_STREAMS; i++)
+        if (h->aux[i]) (void)hipStreamDestroy(h->aux[i]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
