@@ -1139,12 +1139,33 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         if (h->ktab[c].t.nk > 0)
             HIPCHK(h, hfx_launch_ket_gather(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_D, n, h->stream));
     bool done[HFX_NPAIRCLASS][HFX_NPAIRCLASS] = {};
-    for (const auto& c : h->rcombos) {
+    // the row launches dealt largest-first over the streams, as in the Fock build
+    const int ns = h->nstreams;
+    if (ns > 1) {
+        HIPCHK(h, hipEventRecord(h->evaux[0], h->stream));
+        for (int i = 0; i + 1 < ns; i++) HIPCHK(h, hipStreamWaitEvent(h->aux[i], h->evaux[0], 0));
+    }
+    std::vector<std::pair<double, int>> order;
+    for (size_t i = 0; i < h->rcombos.size(); i++)
+        for (int prec = 0; prec < 2; prec++)
+            if (h->rcombos[i].nblk[prec] > 0) order.push_back({h->rcombos[i].cost[prec], (int)(2 * i + prec)});
+    std::stable_sort(order.begin(), order.end(), [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+        return x.first > y.first;
+    });
+    double load[HFX_AUX_STREAMS + 1] = {};
+    for (const auto& o : order) {
+        const auto& c = h->rcombos[o.second / 2];
+        const int prec = o.second % 2;
         int nrow = 0, nab = 0;
         hfx_row_shape(c.cb, c.ck, &nrow, &nab);
         int dlds = c.dlds;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) dlds = 0;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) continue;
+        int si = 0;
+        for (int i = 1; i < ns; i++)
+            if (load[i] < load[si]) si = i;
+        load[si] += o.first;
+        hipStream_t s = si == 0 ? h->stream : h->aux[si - 1];
         GradRowArgs r;
         r.pairs = h->d_pairs;
         r.prims = h->d_prims;
@@ -1163,16 +1184,17 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         r.D = h->d_D;
         r.grad = h->d_grad;
         r.n = n;
-        for (int prec = 0; prec < 2; prec++) {
-            if (c.nblk[prec] == 0) continue;
-            r.kbeg = c.d_kbeg[prec];
-            r.kend = c.d_kend[prec];
-            r.blk_row = c.d_blk_row[prec];
-            r.blk_k0 = c.d_blk_k0[prec];
-            r.chunk = c.chunk[prec];
-            HIPCHK(h, hfx_launch_grad_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], h->stream));
-        }
+        r.kbeg = c.d_kbeg[prec];
+        r.kend = c.d_kend[prec];
+        r.blk_row = c.d_blk_row[prec];
+        r.blk_k0 = c.d_blk_k0[prec];
+        r.chunk = c.chunk[prec];
+        HIPCHK(h, hfx_launch_grad_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], s));
         done[c.cb][c.ck] = true;
+    }
+    for (int i = 0; i + 1 < ns; i++) {
+        HIPCHK(h, hipEventRecord(h->evaux[i + 1], h->aux[i]));
+        HIPCHK(h, hipStreamWaitEvent(h->stream, h->evaux[i + 1], 0));
     }
     GradArgs a;
     a.pairs = h->d_pairs;
