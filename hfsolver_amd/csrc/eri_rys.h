@@ -182,6 +182,12 @@ struct KetPrimsAoSG : KetPrimsAoS {
     HFX_HD HFX_INLINE double cexp(int k) const { return pe[k].aa; }
 };
 
+// Rys rule of the FP32 kernels: 0 = the FP64 rule (Boys function in FP64),
+// 1 = the single-precision rules of rys.h.
+#ifndef HFX_FP32_RULES
+#define HFX_FP32_RULES 0
+#endif
+
 // Contracted integrals of the shell quartet (bra | ket), slice (sa, sc):
 // eri[((a*NB + b)*NCt + c)*ND + d] for the thread's a- and c-components.
 // bra: shell pair with l_A = LA, l_B = LB (primitive pairs prims[bra.p0 ..]);
@@ -238,14 +244,26 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const BP* __restrict_
             const PT T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
             const PT pref = PT(P.Kp) * kq * isq;
             PT u[NR], w[NR];
-            if constexpr (NR <= 2) {
-                if constexpr (sizeof(PT) == 4)
+            if constexpr (sizeof(PT) == 4 && HFX_FP32_RULES) {
+                if constexpr (NR <= 2)
                     rys_roots_small_f<NR>(T, u, w, tab.boysf);
                 else
-                    rys_roots_small<NR>(T, u, w, tab.boys);
-            } else {
-                if constexpr (sizeof(PT) == 4)
                     rys_roots_f<NR>(T, u, w, tab);
+            } else if constexpr (sizeof(PT) == 4) {
+                // FP32 quartets: the Boys function and the Rys rule in FP64
+                double ud[NR], wd[NR];
+                if constexpr (NR <= 2)
+                    rys_roots_small<NR>(double(T), ud, wd, tab.boys);
+                else
+                    rys_roots<NR>(double(T), ud, wd, tab);
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    u[r] = PT(ud[r]);
+                    w[r] = PT(wd[r]);
+                }
+            } else {
+                if constexpr (NR <= 2)
+                    rys_roots_small<NR>(T, u, w, tab.boys);
                 else
                     rys_roots<NR>(T, u, w, tab);
             }

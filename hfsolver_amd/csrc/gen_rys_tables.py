@@ -51,7 +51,7 @@ DPS = 60
 # F_0..F_3): F_m(T) = sum_{k<BOYS_K} F_{m+k}(T0) (T0-T)^k / k!, T0 on a grid of
 # step BOYS_DT, |T - T0| <= BOYS_DT/2, error < (1/16)^9/9! ~ 4e-17 relative.
 BOYS_DT = 0.125
-BOYS_TMAX = 36.0
+BOYS_TMAX = 46.0
 BOYS_K = 9
 BOYS_MMAX = 3                      # highest order needed (two roots: F_0..F_3)
 BOYS_NCOL = BOYS_MMAX + BOYS_K     # F_0 .. F_11 per grid point

@@ -76,12 +76,15 @@ HFX_HD HFX_INLINE double hfx_rcp(double x) {
 // F_m(T) for T < BOYS_TMAX: Taylor series about the nearest grid point T0,
 //   F_m(T) = sum_{k < BOYS_K} F_{m+k}(T0) (T0 - T)^k / k!;
 // beyond: F_0 = (1/2) sqrt(pi/T) (erfc(sqrt T) < 1e-16 F_0 there) and the
-// upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable for T > m.
-// For one root beyond BOYS_TMAX, e^{-T} < 2e-15 F_0, so u = F_1/F_0 = 1/(2T)
-// to that relative accuracy and no exponential is evaluated.
-// n = 1: u = F_1/F_0, w = F_0. n = 2: the monic orthogonal quadratic
-// u^2 + a u + b of the moment sequence, roots u_2 = (-a + sqrt(a^2 - 4b))/2,
-// u_1 = b / u_2, weights w_1 = (F_1 - u_2 F_0)/(u_1 - u_2), w_2 = F_0 - w_1.
+// upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), in which e^{-T}
+// (below 1.1e-20 for T >= 46) is under 1e-16 of (2m+1) F_m for m <= 2 and is
+// left out, so that no exponential is evaluated.
+// n = 1: u = F_1/F_0, w = F_0. n = 2: the orthogonal quadratic of the moment
+// sequence, D u^2 + A u + B with D = F_0 F_2 - F_1^2, A = F_1 F_2 - F_0 F_3,
+// B = F_1 F_3 - F_2^2; with E = (-A + sqrt(A^2 - 4 D B))/2 the roots are
+// u_2 = E / D and u_1 = B / E, the weights w_1 = (F_1 - u_2 F_0)/(u_1 - u_2)
+// = E (F_1 D - E F_0) / (B D - E^2) and w_2 = F_0 - w_1. The three quotients
+// share one reciprocal, 1 / (D E (B D - E^2)).
 template <int NR>
 HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const double* boys) {
     static_assert(NR == 1 || NR == 2, "closed form for one or two roots only");
@@ -113,22 +116,22 @@ HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const dou
             u[0] = it2;
             w[0] = F[0];
         } else {
-            const double et = exp(-T);
 #pragma unroll
-            for (int m = 0; m + 1 < NM; m++) F[m + 1] = ((2 * m + 1) * F[m] - et) * it2;
+            for (int m = 0; m + 1 < NM; m++) F[m + 1] = (2 * m + 1) * F[m] * it2;
         }
     }
     if constexpr (NR == 2) {
         const double m0 = F[0], m1 = F[1], m2 = F[2], m3 = F[3];
-        const double id = hfx_rcp(m0 * m2 - m1 * m1);
-        const double a = (m1 * m2 - m0 * m3) * id;
-        const double b = (m1 * m3 - m2 * m2) * id;
-        const double u2 = 0.5 * (-a + sqrt(a * a - 4.0 * b));
-        const double u1 = b * hfx_rcp(u2);
-        const double w1 = (m1 - u2 * m0) * hfx_rcp(u1 - u2);
-        u[0] = u1;
+        const double D = m0 * m2 - m1 * m1;
+        const double A = m1 * m2 - m0 * m3;
+        const double B = m1 * m3 - m2 * m2;
+        const double E = 0.5 * (-A + sqrt(A * A - 4.0 * D * B));
+        const double G = B * D - E * E;
+        const double r = hfx_rcp(D * E * G);
+        u[0] = B * D * G * r;
+        u[1] = E * E * G * r;
+        const double w1 = (m1 * D - E * m0) * E * D * E * r;
         w[0] = w1;
-        u[1] = u2;
         w[1] = m0 - w1;
     }
 }

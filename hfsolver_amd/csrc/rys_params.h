@@ -11,11 +11,11 @@
 #define RYS_CHEBF_SIZE 12600
 #define BOYS_DT 0.125
 #define BOYS_INV_DT 8
-#define BOYS_TMAX 36
+#define BOYS_TMAX 46
 #define BOYS_K 9
 #define BOYS_NCOL 12
-#define BOYS_NPT 289
-#define BOYS_SIZE 3468
+#define BOYS_NPT 369
+#define BOYS_SIZE 4428
 #define BOYSF_K 5
 #define BOYSF_NCOL 8
-#define BOYSF_SIZE 2312
+#define BOYSF_SIZE 2952

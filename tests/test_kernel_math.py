@@ -80,7 +80,7 @@ def test_rys_rule_reproduces_boys_moments(lib):
     w = np.zeros(5)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     for nr in range(1, 6):
-        for T in [0.0, 1e-8, 0.37, 1.0, 4.5, 17.3, 33.0, 35.99, 36.0, 59.99, 60.0, 85.0, 1e3]:
+        for T in [0.0, 1e-8, 0.37, 1.0, 4.5, 17.3, 33.0, 35.99, 36.0, 45.99, 46.0, 59.99, 60.0, 85.0, 1e3, 1e5]:
             lib.chk_rys(nr, ctypes.c_double(T), p(u), p(w))
             F = O.boys(2 * nr - 1, T)
             for k in range(2 * nr):
