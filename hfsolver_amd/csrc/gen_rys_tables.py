@@ -18,7 +18,12 @@ Method (all published, nothing tabulated by hand):
     harmless at 60 digits;
   * nodes and weights from the Jacobi matrix (Golub-Welsch);
   * each u_i(T), w_i(T) is fitted on [0, TMAX) by piecewise Chebyshev
-    interpolation (NINT intervals of width H, degree DEG);
+    interpolation (NINT intervals of width H, degree DEG), and the series is
+    rewritten in powers of the interval variable x in [-1, 1] (in 60-digit
+    arithmetic, then rounded): the kernels evaluate it by Horner's rule, one
+    multiply-add per degree; the Chebyshev coefficients decay fast enough that
+    sum_k |a_k| stays below 2.4 |f(x)| on every interval, so the power form
+    loses at most a few ulps;
   * for T >= TMAX the weight exp(-T t^2) is negligible at t = 1 and the rule
     becomes half-range Gauss-Hermite: u_i = x_i^2 / T, w_i = W_i / sqrt(T)
     with x_i > 0 the positive nodes of the 2n-point Gauss-Hermite rule and W_i
@@ -26,7 +31,7 @@ Method (all published, nothing tabulated by hand):
 
 Output: rys_params.h (sizes) and rys_tables.inc (C/HIP source, doubles), layout
 
-    RYS_CHEB[n][interval][coef k][2*i + {0: u, 1: w}]   (flattened, see offsets)
+    RYS_POLY[n][interval][power k][2*i + {0: u, 1: w}]   (flattened, see offsets)
 
 so that one lane evaluating root i of one interval reads 2*DEG+2 doubles
 that sit in one contiguous stretch per coefficient row.
@@ -119,7 +124,7 @@ def fit_interval(args):
         xs.append(x)
         T = a + H * (x + 1) / 2
         vals.append(rys_exact(n, T))
-    coefs = []  # coefs[k][2*i + s]
+    cheb = []  # cheb[k][2*i + s], 60 digits
     for k in range(DEG + 1):
         row = []
         for i in range(n):
@@ -130,9 +135,21 @@ def fit_interval(args):
                 acc *= mp.mpf(2) / (DEG + 1)
                 if k == 0:
                     acc /= 2
-                row.append(float(acc))
-        coefs.append(row)
-    return n, it, coefs
+                row.append(acc)
+        cheb.append(row)
+    # power form: coefficients of the Chebyshev polynomials T_k(x) by
+    # T_{k+1} = 2x T_k - T_{k-1}, then a_m = sum_k c_k [x^m] T_k
+    tk = [[mp.mpf(1)], [mp.mpf(0), mp.mpf(1)]]
+    for k in range(2, DEG + 1):
+        nxt = [mp.mpf(0)] * (k + 1)
+        for m, v in enumerate(tk[k - 1]):
+            nxt[m + 1] += 2 * v
+        for m, v in enumerate(tk[k - 2]):
+            nxt[m] -= v
+        tk.append(nxt)
+    poly = [[float(sum(cheb[k][f] * (tk[k][m] if m < len(tk[k]) else 0) for k in range(DEG + 1)))
+             for f in range(2 * n)] for m in range(DEG + 1)]
+    return n, it, [[float(v) for v in row] for row in cheb], poly
 
 
 def main():
@@ -140,19 +157,20 @@ def main():
     jobs = [(n, it) for n in range(1, NMAX + 1) for it in range(NINT)]
     with Pool(min(8, os.cpu_count() or 1)) as pool:
         res = pool.map(fit_interval, jobs)
-    table = {(n, it): c for n, it, c in res}
+    table = {(n, it): c for n, it, c, p in res}
+    ptable = {(n, it): p for n, it, c, p in res}
     o = 0
     for n in range(1, NMAX + 1):
         o += NINT * (DEG + 1) * 2 * n
     hdr = ["// Generated by gen_rys_tables.py - do not edit.",
-           "// Piecewise-Chebyshev Rys nodes/weights on T in [0, %g), Gauss-Hermite beyond." % TMAX,
+           "// Piecewise-polynomial Rys nodes/weights on T in [0, %g), Gauss-Hermite beyond." % TMAX,
            "#pragma once",
            "#define RYS_NMAX %d" % NMAX,
            "#define RYS_NINT %d" % NINT,
            "#define RYS_DEG %d" % DEG,
            "#define RYS_H %.17g" % H,
            "#define RYS_TMAX %.17g" % TMAX,
-           "#define RYS_CHEB_SIZE %d" % o,
+           "#define RYS_POLY_SIZE %d" % o,
            "#define RYS_DEGF %d" % DEGF,
            "#define RYS_CHEBF_SIZE %d" % (o // (DEG + 1) * (DEGF + 1)),
            "#define BOYS_DT %.17g" % BOYS_DT,
@@ -169,11 +187,11 @@ def main():
         f.write("\n".join(hdr) + "\n")
     lines = ["// Generated by gen_rys_tables.py - do not edit. Data of rys_params.h;",
              "// define RYS_TABLE_QUALIFIER before including."]
-    lines.append("RYS_TABLE_QUALIFIER double RYS_CHEB[RYS_CHEB_SIZE] = {")
+    lines.append("RYS_TABLE_QUALIFIER double RYS_POLY[RYS_POLY_SIZE] = {")
     for n in range(1, NMAX + 1):
         lines.append("// n = %d" % n)
         for it in range(NINT):
-            for row in table[(n, it)]:
+            for row in ptable[(n, it)]:
                 lines.append("  " + ", ".join("%.17e" % v for v in row) + ",")
     lines.append("};")
     # single-precision truncation for the FP32 kernels, same layout with RYS_DEGF + 1 coefficients

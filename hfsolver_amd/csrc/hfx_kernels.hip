@@ -34,7 +34,7 @@ namespace hfx_dev {
 namespace {
 
 __device__ HFX_INLINE RysTables dev_tables(const double* boys = hfx_dev::BOYS_TAB) {
-    return RysTables{hfx_dev::RYS_CHEB, hfx_dev::RYS_HERM_U, hfx_dev::RYS_HERM_W, boys, hfx_dev::BOYSF_TAB,
+    return RysTables{hfx_dev::RYS_POLY, hfx_dev::RYS_HERM_U, hfx_dev::RYS_HERM_W, boys, hfx_dev::BOYSF_TAB,
                      hfx_dev::RYS_CHEBF};
 }
 

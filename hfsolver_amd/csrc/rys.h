@@ -13,7 +13,7 @@
 
 // Pointers to the table data (device arrays in kernels, host arrays on the CPU).
 struct RysTables {
-    const double* cheb;  // RYS_CHEB
+    const double* poly;  // RYS_POLY
     const double* hu;    // RYS_HERM_U
     const double* hw;    // RYS_HERM_W
     const double* boys;  // BOYS_TAB (global memory, or a copy in LDS)
@@ -194,7 +194,7 @@ HFX_HD HFX_INLINE void rys_roots_small_f(float T, float* u, float* w, const floa
     }
 }
 
-// Offset (in doubles) of the n-root block of RYS_CHEB.
+// Offset (in doubles) of the n-root block of RYS_POLY.
 HFX_HD constexpr int rys_offset(int n) {
     int o = 0;
     for (int m = 1; m < n; m++) o += RYS_NINT * (RYS_DEG + 1) * 2 * m;
@@ -217,27 +217,19 @@ HFX_HD HFX_INLINE void rys_roots(double T, double* u, double* w, const RysTables
     int itv = (int)(T * (1.0 / RYS_H));
     if (itv > RYS_NINT - 1) itv = RYS_NINT - 1;
     const double x = 2.0 * (T - itv * RYS_H) * (1.0 / RYS_H) - 1.0;
-    const double x2 = 2.0 * x;
-    const double* c = tab.cheb + rys_offset(NR) + itv * (RYS_DEG + 1) * 2 * NR;
-    double b1[2 * NR], b2[2 * NR];
+    const double* c = tab.poly + rys_offset(NR) + itv * (RYS_DEG + 1) * 2 * NR;
+    double acc[2 * NR];
 #pragma unroll
-    for (int f = 0; f < 2 * NR; f++) {
-        b1[f] = 0.0;
-        b2[f] = 0.0;
-    }
+    for (int f = 0; f < 2 * NR; f++) acc[f] = c[RYS_DEG * 2 * NR + f];
 #pragma unroll
-    for (int k = RYS_DEG; k >= 1; k--) {
+    for (int k = RYS_DEG - 1; k >= 0; k--) {
 #pragma unroll
-        for (int f = 0; f < 2 * NR; f++) {
-            const double t = c[k * 2 * NR + f] + x2 * b1[f] - b2[f];
-            b2[f] = b1[f];
-            b1[f] = t;
-        }
+        for (int f = 0; f < 2 * NR; f++) acc[f] = fma(acc[f], x, c[k * 2 * NR + f]);
     }
 #pragma unroll
     for (int i = 0; i < NR; i++) {
-        u[i] = c[2 * i] + x * b1[2 * i] - b2[2 * i];
-        w[i] = c[2 * i + 1] + x * b1[2 * i + 1] - b2[2 * i + 1];
+        u[i] = acc[2 * i];
+        w[i] = acc[2 * i + 1];
     }
 }
 

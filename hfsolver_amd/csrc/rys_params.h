@@ -1,12 +1,12 @@
 // Generated by gen_rys_tables.py - do not edit.
-// Piecewise-Chebyshev Rys nodes/weights on T in [0, 60), Gauss-Hermite beyond.
+// Piecewise-polynomial Rys nodes/weights on T in [0, 60), Gauss-Hermite beyond.
 #pragma once
 #define RYS_NMAX 5
 #define RYS_NINT 60
 #define RYS_DEG 13
 #define RYS_H 1
 #define RYS_TMAX 60
-#define RYS_CHEB_SIZE 25200
+#define RYS_POLY_SIZE 25200
 #define RYS_DEGF 6
 #define RYS_CHEBF_SIZE 12600
 #define BOYS_DT 0.125
