@@ -16,7 +16,7 @@ namespace chk_tables {
 namespace {
 
 RysTables tabs() {
-    return RysTables{chk_tables::RYS_CHEB, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W, chk_tables::BOYS_TAB,
+    return RysTables{chk_tables::RYS_POLY, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W, chk_tables::BOYS_TAB,
                      chk_tables::BOYSF_TAB, chk_tables::RYS_CHEBF};
 }
 
