@@ -40,12 +40,15 @@ struct QClass {
 };
 
 // Split policy of the engine: slice over a (then over c) while a thread would
-// hold more than 81 integrals.
+// hold more than HFX_JK_SPLIT integrals (81; 54 and 27 measured slower).
+#ifndef HFX_JK_SPLIT
+#define HFX_JK_SPLIT 81
+#endif
 HFX_HD constexpr bool split_a(int la, int lb, int lc, int ld) {
-    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > 81;
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > HFX_JK_SPLIT;
 }
 HFX_HD constexpr bool split_c(int la, int lb, int lc, int ld) {
-    return split_a(la, lb, lc, ld) && ncart(lb) * ncart(lc) * ncart(ld) > 81;
+    return split_a(la, lb, lc, ld) && ncart(lb) * ncart(lc) * ncart(ld) > HFX_JK_SPLIT;
 }
 HFX_HD constexpr int nsplit(int la, int lb, int lc, int ld) {
     return (split_a(la, lb, lc, ld) ? ncart(la) : 1) * (split_c(la, lb, lc, ld) ? ncart(lc) : 1);
