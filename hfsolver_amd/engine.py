@@ -143,7 +143,12 @@ class Engine:
         return G, J, K, st
 
     def fock_2e_device(self, dP, dG, dJ=None, dK=None):
-        """Device-pointer form: dP/dG/dJ/dK are torch CUDA (HIP) float64 tensors of n*n elements."""
+        """Device-pointer form: dP/dG/dJ/dK are torch CUDA (HIP) float64 tensors of n*n elements.
+
+        The engine runs on streams of its own, so the torch stream that produced
+        dP is synchronised first; the outputs are complete on return."""
+        import torch
+        torch.cuda.current_stream(dP.device).synchronize()
         st = HfxStats()
         ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
         self._check(lib().hfx_fock_2e_device(self._h, ptr(dP), ptr(dG), ptr(dJ), ptr(dK), ctypes.byref(st)))

@@ -84,14 +84,16 @@ int hfx_fock_2e(hfx_t *h, const double *P, double *G, double *J, double *K, hfx_
 /*
  * Same on device buffers (HIP device pointers of n*n doubles, resident in
  * HBM): no host transfers. Used by benchmarks and by callers that keep the
- * density on the GPU. Synchronous.
+ * density on the GPU. The engine works on streams of its own: the writes that
+ * produced dP must be complete when it is called (synchronise the producing
+ * stream first). Synchronous: dG, dJ, dK are complete on return.
  */
 int hfx_fock_2e_device(hfx_t *h, const double *dP, double *dG, double *dJ, double *dK, hfx_stats *stats);
 
 /*
  * One-electron matrices S, T, V (n x n, any may be NULL) for the current
  * geometry. Counterpart of getS/getT/getV (src/qc.c:6-93, src/qc.h:13-33);
- * V includes the nuclear charges atno. Evaluated on the host (they are O(n^2)).
+ * V includes the nuclear charges atno. Evaluated on the GPU (one_e_kernel).
  */
 int hfx_one_electron(hfx_t *h, double *S, double *T, double *V);
 
@@ -105,7 +107,7 @@ int hfx_grad_2e(hfx_t *h, const double *P, double *g);
 /*
  * One-electron part of the RHF gradient:
  *   g_X = sum_ij P_ij dH_ij/dX - sum_ij W_ij dS_ij/dX  (Hellmann-Feynman term included),
- * W = energy-weighted density. g: 3*nat. Evaluated on the host.
+ * W = energy-weighted density. g: 3*nat. Evaluated on the GPU (one_e_kernel).
  */
 int hfx_grad_1e(hfx_t *h, const double *P, const double *W, double *g);
 
