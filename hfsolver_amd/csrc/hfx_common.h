@@ -90,6 +90,14 @@ HFX_HD HFX_INLINE int hfx_ld_int(const int* p) {
 #endif
 }
 
+HFX_HD HFX_INLINE double hfx_ld_dbl(const double* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const HFX_AS4 double*)p;
+#else
+    return *p;
+#endif
+}
+
 HFX_HD HFX_INLINE PrimPair hfx_ld_prim(const PrimPair* p) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const HFX_AS4 double* d = (const HFX_AS4 double*)p;

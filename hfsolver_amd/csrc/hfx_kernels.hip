@@ -139,6 +139,11 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 #endif
 
 #if HFX_PART == 2 || HFX_PART == 3
+// J_ab of one-row blocks summed in registers when the bra block has at most
+// this many elements (otherwise reduced over the wavefront per quartet)
+#ifndef HFX_JAB_REG_MAX
+#define HFX_JAB_REG_MAX 36
+#endif
 // Row mode: one block = R bra shell pairs x a chunk of the ket table (see RowArgs).
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
@@ -185,8 +190,8 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
         bprims = a.prims;
     // one-row blocks (the usual case): the bra's D_ab loaded once, its J_ab summed
     // in registers over all kets of the block and reduced over the wavefront at the end
-    const bool onerow = nr == 1;
-    R dab0[NAt][NB];
+    const bool onerow = nr == 1 && NAt * NB <= HFX_JAB_REG_MAX;
+    double dab0[NAt][NB];   // wavefront-uniform: scalar loads
     double jab[NAt][NB];
     {
         const ShellPair bra0 = hfx_ld_pair(a.pairs + hfx_ld_int(a.bra_list + row0));
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
         for (int i = 0; i < NAt; i++)
 #pragma unroll
             for (int j = 0; j < NB; j++) {
-                dab0[i][j] = R(a.D[(size_t)(bra0.fa + sa + i) * n + bra0.fb + j]);
+                dab0[i][j] = hfx_ld_dbl(a.D + (size_t)(bra0.fa + sa + i) * n + bra0.fb + j);
                 jab[i][j] = 0.0;
             }
     }
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(256) void jk_row_kernel(RowArgs a) {
 #pragma unroll
                         for (int i = 0; i < NAt; i++)
 #pragma unroll
-                            for (int j = 0; j < NB; j++) dn.ab[i][j] = dab0[i][j];
+                            for (int j = 0; j < NB; j++) dn.ab[i][j] = R(dab0[i][j]);
                     } else {
 #pragma unroll
                         for (int i = 0; i < NAt; i++)
