@@ -21,9 +21,15 @@ struct RysTables {
     const float* chebf;  // RYS_CHEBF, single-precision truncated Chebyshev series of the FP32 kernels
 };
 
+// 1/sqrt(x) for positive normal x (p + q, or T beyond the tables): on the device
+// the hardware estimate refined by one Householder step of order 3,
+// r <- r + r e (1/2 + 3/8 e) with e = 1 - x r^2, which takes the estimate's
+// relative error eps to about (5/16) eps^3.
 HFX_HD HFX_INLINE double hfx_rsqrt(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return ::rsqrt(x);
+    const double r = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * r, r, 1.0);
+    return fma(r * e, fma(e, 0.375, 0.5), r);
 #else
     return 1.0 / std::sqrt(x);
 #endif
@@ -36,6 +42,15 @@ HFX_HD HFX_INLINE float hfx_rsqrt(float x) {
     return __builtin_amdgcn_rsqf(x);
 #else
     return 1.0f / std::sqrt(x);
+#endif
+}
+
+// i * j for i, j < 2^24 (table offsets): the 24-bit multiply on the device.
+HFX_HD HFX_INLINE unsigned hfx_umul24(unsigned i, unsigned j) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(i, j);
+#else
+    return i * j;
 #endif
 }
 
@@ -91,12 +106,12 @@ HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const dou
     constexpr int NM = 2 * NR;  // moments F_0 .. F_{2NR-1}
     double F[NM];
     if (T < BOYS_TMAX) {
-        const int i = (int)(T * BOYS_INV_DT + 0.5);
+        const unsigned i = (unsigned)(T * BOYS_INV_DT + 0.5);
         const double x = i * BOYS_DT - T;
         double xk[BOYS_K];
 #pragma unroll
         for (int k = 1; k < BOYS_K; k++) xk[k] = x * (1.0 / k);
-        const double* row = boys + i * BOYS_NCOL;
+        const double* row = boys + hfx_umul24(i, BOYS_NCOL);
 #pragma unroll
         for (int m = 0; m < NM; m++) {
             double acc = row[m + BOYS_K - 1];
