@@ -102,6 +102,10 @@ def main():
     ap.add_argument("--fp32-threshold", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true", help="skip the analytic-gradient timing")
+    ap.add_argument("--forces", type=int, default=0, metavar="N",
+                    help="also run N consecutive energy+force evaluations (SCF warm-started from the previous "
+                         "density) on seeded +-0.01 A perturbations, mixed engine vs FP64 engine (10 in the "
+                         "mixed-precision force workload); reported under 'forces'")
     ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known bytes) to scale HBM counters")
@@ -220,6 +224,41 @@ def main():
         grad = {"grad_2e_seconds_fp64": t2, "grad_2e_seconds_mixed": t2mx, "grad_1e_seconds": t1,
                 "gradients_per_s_mixed": 1.0 / (t2mx + t1)}
 
+    forces = None
+    if args.forces > 0:
+        from hfsolver_amd.clusters import ANG2BOHR
+        from hfsolver_amd.grad import ForceField
+        rng = np.random.Generator(np.random.PCG64(args.seed + 1))
+        kw = dict(Nscf=80, tolE=1e-10, tolP=1e-7, diis=True, eig="device")
+        ffm = ForceField(atno, xyz, fp32_threshold=args.fp32_threshold, **kw)
+        ff64 = ForceField(atno, xyz, fp32_threshold=0.0, **kw)
+        if r64 is not None:   # the first evaluation starts from the base geometry's density
+            ffm.P = ff64.P = np.asarray(r64["P"])
+        tm, gmax, grms, dEs, its = [], [], [], [], []
+        for k in range(args.forces):
+            x = xyz + rng.uniform(-0.01, 0.01, size=xyz.shape) * ANG2BOHR
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            Em, gm = ffm(x)
+            torch.cuda.synchronize()
+            tm.append(time.perf_counter() - t0)
+            E64, g64 = ff64(x)
+            d = np.asarray(gm) - np.asarray(g64)
+            gmax.append(float(np.abs(d).max()))
+            grms.append(float(np.sqrt(np.mean(d * d))))
+            dEs.append(abs(Em - E64))
+            its.append(int(ffm.last.iterations))
+            log("  forces %d: mixed SCF+gradient %.2f s (%d SCF iterations), |dE| %.2e Eh, max|dg| %.2e, rms %.2e Eh/bohr"
+                % (k, tm[-1], its[-1], dEs[-1], gmax[-1], grms[-1]))
+        ffm.close()
+        ff64.close()
+        forces = {"evaluations": args.forces, "components": int(3 * len(atno)),
+                  "gradients_per_s_mixed": args.forces / sum(tm), "seconds_per_evaluation_mixed": sum(tm) / args.forces,
+                  "scf_iterations_mixed": its, "force_error_max_vs_fp64": max(gmax),
+                  "force_error_rms_vs_fp64": float(np.sqrt(np.mean(np.square(grms)))),
+                  "abs_dE_max_vs_fp64": max(dEs),
+                  "perturbation": "uniform +-0.01 A per coordinate about the base geometry, seed %d" % (args.seed + 1)}
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(e64, b, st64[0].quartets_fp64)
 
     jk64 = np.mean([s.jk_seconds for s in st64])
@@ -263,6 +302,7 @@ def main():
         "mixed_max_dF": dF,
         "E_elec_fp64_at_density": E64,
         "gradient": grad,
+        "forces": forces,
         "roofline": {"bound": "fp64-valu", "achieved": roof64 / 1e12, "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": roof64 / PEAK_FP64, "traffic": traffic,
                      "kernel": "jk_row_kernel/jk_kernel (all class launches of one FP64 build)",
