@@ -115,6 +115,28 @@ def test_mixed_precision_dimer(golden):
     e32.close()
 
 
+def test_stream_count_does_not_change_results(monkeypatch):
+    """The class launches dealt over 1 or 3 streams give the same J and K
+    (FP64 atomics in another order: agreement to rounding)."""
+    atno, xyz = water_cluster(8, seed=8)
+    b = Basis(atno, xyz)
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((b.nbf, b.nbf)) * 0.1
+    P = X + X.T
+    out = []
+    for ns in ("1", "3"):
+        monkeypatch.setenv("HFX_STREAMS", ns)
+        for thr in (0.0, 1e-5):
+            say("streams", ns, "fp32 below", thr)
+            eng = Engine(b, fp32_threshold=thr)
+            G, J, K, _ = eng.fock_2e(P, want_jk=True)
+            eng.close()
+            out.append((J, K))
+    for (J1, K1), (J3, K3) in zip(out[:2], out[2:]):
+        assert np.abs(J3 - J1).max() <= 1e-12 * np.abs(J1).max()
+        assert np.abs(K3 - K1).max() <= 1e-12 * np.abs(K1).max()
+
+
 def test_sixteen_waters_invariants():
     atno, xyz = water_cluster(16, seed=16)
     say("16 waters: build")
