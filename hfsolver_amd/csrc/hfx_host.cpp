@@ -120,6 +120,7 @@ struct hfx_engine {
     // tuning of the row-mode blocks (HFX_ROW_RMAX, HFX_ROW_PAIR_KB, HFX_ROW_DLDS)
     int row_rmax = HFX_ROWS_MAX;
     size_t row_pair_bytes = 32 * 1024;
+    double row_blocks = 128;   // blocks a launch is cut into when its rows are few (HFX_ROW_BLOCKS)
     int row_dlds = 1;
     double npairs_sig = 0;
     // device memory
@@ -416,10 +417,12 @@ int compute_schwarz(hfx_t* h) {
     return 0;
 }
 
-// Ket positions per block: whole ranges when there are enough row groups to
-// fill the device, otherwise about 4096 blocks' worth (multiples of 256).
-int block_chunk(double tot, int ngroups) {
-    const double target_blocks = 4096;
+// Ket positions per block: whole ranges when a launch has at least
+// target_blocks row groups, otherwise about target_blocks blocks' worth
+// (multiples of 256). Every block stages its rows' density and flushes its K
+// rows, so cutting rows into more blocks costs traffic; the concurrent launches
+// of the other streams keep the device busy instead.
+int block_chunk(double tot, int ngroups, double target_blocks = 128) {
     if (ngroups >= target_blocks) return 1 << 20;
     return std::max(256, (int)std::ceil(tot / target_blocks / 256.0) * 256);
 }
@@ -686,7 +689,7 @@ int build_lists(hfx_t* h) {
                     }
                 }
                 if (ngroups == 0) continue;
-                const int chunk = block_chunk(tot, ngroups);
+                const int chunk = block_chunk(tot, ngroups, h->row_blocks);
                 std::vector<int> br, bk;
                 std::vector<std::pair<int, int>> rng;
                 for (int g0 = 0; g0 < nb; g0 += R) {
@@ -888,6 +891,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (const char* e = std::getenv("HFX_ROW_RMAX")) h->row_rmax = std::max(1, std::min(HFX_ROWS_MAX, std::atoi(e)));
     if (const char* e = std::getenv("HFX_ROW_PAIR_KB")) h->row_pair_bytes = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
+    if (const char* e = std::getenv("HFX_ROW_BLOCKS")) h->row_blocks = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("HFX_STREAMS")) h->nstreams = std::max(1, std::min(HFX_AUX_STREAMS + 1, std::atoi(e)));
     h->nat = nat;
     h->atno.assign(atno, atno + nat);

@@ -209,6 +209,9 @@ struct BlockArgs {
 };
 
 static constexpr int HFX_ROWS_MAX = 4;   // bra rows per row-mode block
+#ifndef HFX_ROW_THREADS
+#define HFX_ROW_THREADS 256                // threads of a J/K row-mode block
+#endif
 static constexpr int HFX_AUX_STREAMS = 3; // extra streams of the class launches (at most)
 static constexpr size_t HFX_ROW_LDS_MAX = 128 * 1024;   // dynamic LDS of a row-mode block
 static constexpr int HFX_TILE_ROWS = 4;   // one bra row per wavefront
