@@ -121,6 +121,11 @@ struct hfx_engine {
     int row_rmax = HFX_ROWS_MAX;
     size_t row_pair_bytes = 32 * 1024;
     double row_blocks = 128;   // blocks a launch is cut into when its rows are few (HFX_ROW_BLOCKS)
+    // density-weighted screening of the Fock builds (hfx_set_density_screening)
+    double dthr = 0.0;
+    float* d_dsh = nullptr;
+    int* d_dsh_f0 = nullptr;
+    int* d_dsh_nc = nullptr;
     int row_dlds = 1;
     double npairs_sig = 0;
     // device memory
@@ -459,6 +464,8 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
     t.nk = nk;
     std::vector<KetInfo> info(nk);
     std::vector<double> cd(3 * (size_t)nk), prim((size_t)ps);
+    std::vector<int> shcd(nk);
+    std::vector<float> qk(nk);
     for (int s = 0; s < S; s++) {
         const int nks = (int)subl[s].size();
         for (int j = 0; j < nks; j++) {
@@ -466,6 +473,8 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
             const int k = t.off[s] + j;
             if (sp.cpa >= 32768 || sp.cpb >= 32768) return fail(h, "ket table: too many functions of one angular momentum");
             info[k] = KetInfo{subl[s][j], sp.fa, sp.fb, sp.cpa | (sp.cpb << 15) | ((sp.A == sp.B ? 1 : 0) << 30)};
+            shcd[k] = (sp.A & 0xffff) | (sp.B << 16);
+            qk[k] = (float)(sp.Q * (1.0 + 1e-6));
             cd[k] = sp.ABx;
             cd[(size_t)nk + k] = sp.ABy;
             cd[2 * (size_t)nk + k] = sp.ABz;
@@ -487,6 +496,12 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
     if (upload(h, &d_cd, cd, true)) return 1;
     if (upload(h, &d_prim, prim, true)) return 1;
     if (upload(h, &d_primf, primf, true)) return 1;
+    int* d_shcd = nullptr;
+    float* d_qk = nullptr;
+    if (upload(h, &d_shcd, shcd, true)) return 1;
+    if (upload(h, &d_qk, qk, true)) return 1;
+    t.shcd = d_shcd;
+    t.qk = d_qk;
     if (dmalloc(h, (void**)&t.Dcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
     if (dmalloc(h, (void**)&t.Jcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
     t.info = d_info;
@@ -941,6 +956,9 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     const int n = h->nbf;
     HIPCHK(h, hfx_launch_prep(dP, h->d_scale, h->d_D, h->d_Jb, h->d_Kb, n, h->stream));
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    const int nsh = (int)h->shells.size();
+    if (h->dthr > 0)
+        HIPCHK(h, hfx_launch_shell_dmax(h->d_D, n, h->d_dsh_f0, h->d_dsh_nc, nsh, h->d_dsh, h->stream));
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
         if (h->ktab[c].t.nk > 0)
             HIPCHK(h, hfx_launch_ket_gather(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_D, n, h->stream));
@@ -992,6 +1010,9 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.J = h->d_Jb;
             r.K = h->d_Kb;
             r.n = n;
+            r.dsh = h->dthr > 0 ? h->d_dsh : nullptr;
+            r.dthr = (float)h->dthr;
+            r.nsh = nsh;
             HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], s));
         }
     }
@@ -1414,6 +1435,25 @@ int hfx_pair_bounds(const hfx_t* h, int* npairs, int* A, int* B, double* Q) {
     return 0;
 }
 
+int hfx_set_density_screening(hfx_t* h, double threshold) {
+    if (!h) return 1;
+    if (!(threshold >= 0)) return fail(h, "hfx_set_density_screening: threshold must be >= 0");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int nsh = (int)h->shells.size();
+    if (threshold > 0 && nsh >= 65536) return fail(h, "hfx_set_density_screening: more than 65535 shells");
+    if (threshold > 0 && !h->d_dsh) {
+        std::vector<int> f0(nsh), nc(nsh);
+        for (int i = 0; i < nsh; i++) {
+            f0[i] = h->shells[i].f0;
+            nc[i] = ncart(h->shells[i].L);
+        }
+        if (upload(h, &h->d_dsh_f0, f0, false) || upload(h, &h->d_dsh_nc, nc, false)) return 1;
+        if (dmalloc(h, (void**)&h->d_dsh, (size_t)nsh * nsh * sizeof(float), false)) return 1;
+    }
+    h->dthr = threshold;
+    return 0;
+}
+
 int hfx_screening(const hfx_t* h, double* npairs, double* nq64, double* nq32) {
     if (!h) return 1;
     double a = 0, b = 0;
@@ -1442,7 +1482,8 @@ void hfx_destroy(hfx_t* h) {
                 if (h->oel[o][la][lb].B) (void)hipFree(h->oel[o][la][lb].B);
             }
     void* bufs[] = {h->d_scale, h->d_D, h->d_Jb, h->d_Kb, h->d_P, h->d_G, h->d_J, h->d_K, h->d_shell_atom, h->d_bf_atom,
-                    h->d_grad, h->d_sh_atom, h->d_sh_f0, h->d_sh_nprim, h->d_sh_prim0, h->d_alpha, h->d_coef, h->d_Z};
+                    h->d_grad, h->d_sh_atom, h->d_sh_f0, h->d_sh_nprim, h->d_sh_prim0, h->d_alpha, h->d_coef, h->d_Z,
+                    h->d_dsh, h->d_dsh_f0, h->d_dsh_nc};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);

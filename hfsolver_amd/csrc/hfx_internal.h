@@ -52,6 +52,8 @@ struct KetTab {
     const float* primf;   // the same in single precision (FP32 kernels)
     double* Dcd;          // [NC * ND][nk]: density blocks D_cd, gathered per build
     double* Jcd;          // [NC * ND][nk]: J_cd accumulators, scattered into J after the build
+    const int* shcd;      // [nk]: shells C | D << 16 (density-weighted screening)
+    const float* qk;      // [nk]: Schwarz bound of the ket pair (density-weighted screening)
     int nk;
     int S;
     int off[HFX_SUB_MAX + 1];
@@ -93,6 +95,12 @@ struct RowArgs {
     double* J;
     double* K;
     int n;
+    // density-weighted screening (dsh == nullptr: off): a quartet is skipped when
+    // Q_ab Q_cd max(dsh over its six shell-pair blocks) < dthr, dsh[i * nsh + j]
+    // = max |D| over the block of shells i, j
+    const float* dsh;
+    float dthr;
+    int nsh;
 };
 
 // Bytes of dynamic LDS of a row-mode launch: K rows [R][nrow][ncols] and J_ab
@@ -245,5 +253,8 @@ hipError_t hfx_launch_ket_gather(const KetTab& kt, int ncomp, int nd, const doub
 hipError_t hfx_launch_ket_scatter(const KetTab& kt, int ncomp, int nd, double* Jb, int n, hipStream_t s);
 hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* scale, double* G, double* J, double* K,
                              int n, hipStream_t s);
+// dsh[i * nsh + j] = max |D| over the block of shells i, j (rounded up to float)
+hipError_t hfx_launch_shell_dmax(const double* D, int n, const int* sh_f0, const int* sh_nc, int nsh, float* dsh,
+                                 hipStream_t s);
 int hfx_nsplit(int cls_bra, int cls_ket);
 hipError_t hfx_launch_one_e(int la, int lb, bool deriv, const OneEArgs& a, hipStream_t s);

@@ -248,8 +248,17 @@ __global__ __launch_bounds__(HFX_ROW_THREADS) void jk_row_kernel(RowArgs a) {
                 const int row = row0 + r;
                 const int bp = hfx_ld_int(a.bra_list + row);
                 const ShellPair bra = hfx_ld_pair(a.pairs + bp);
-                const bool act = incol && col >= hfx_ld_int(a.kbeg + row * kt.S + s) &&
-                                 col < hfx_ld_int(a.kend + row * kt.S + s);
+                bool act = incol && col >= hfx_ld_int(a.kbeg + row * kt.S + s) &&
+                           col < hfx_ld_int(a.kend + row * kt.S + s);
+                if (a.dsh != nullptr && act) {
+                    // density-weighted screening: the largest density factor of the quartet
+                    const int shk = kt.shcd[col], shc = shk & 0xffff, shd = shk >> 16;
+                    const float* dA = a.dsh + (size_t)bra.A * a.nsh;
+                    const float* dB = a.dsh + (size_t)bra.B * a.nsh;
+                    const float m = fmaxf(fmaxf(fmaxf(dA[shc], dA[shd]), fmaxf(dB[shc], dB[shd])),
+                                          fmaxf(dA[bra.B], a.dsh[(size_t)shc * a.nsh + shd]));
+                    if ((float)bra.Q * kt.qk[col] * m < a.dthr) act = false;
+                }
                 const int fa = bra.fa + sa, fb = bra.fb;
                 JKAcc<LA, LB, LC, LD, SPA, SPC, R> acc;
                 jk_zero(acc);
@@ -998,6 +1007,30 @@ hipError_t hfx_launch_ket_scatter(const KetTab& kt, int ncomp, int nd, double* J
     unsigned nb = (unsigned)((tot + 255) / 256);
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(ket_scatter_kernel, dim3(nb), dim3(256), 0, s, kt, ncomp, nd, Jb, n);
+    return hipGetLastError();
+}
+
+// One thread per shell pair (i, j): the largest |D| of the block, rounded up to
+// float so that the screening test never drops a quartet it should keep.
+__global__ __launch_bounds__(256) void shell_dmax_kernel(const double* __restrict__ D, int n,
+                                                         const int* __restrict__ f0, const int* __restrict__ nc,
+                                                         int nsh, float* __restrict__ dsh) {
+    const size_t tot = (size_t)nsh * nsh;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(k / nsh), j = (int)(k - (size_t)i * nsh);
+        double m = 0.0;
+        for (int a = 0; a < nc[i]; a++)
+            for (int b = 0; b < nc[j]; b++) m = fmax(m, fabs(D[(size_t)(f0[i] + a) * n + f0[j] + b]));
+        dsh[k] = (float)(m * (1.0 + 1e-6));
+    }
+}
+
+hipError_t hfx_launch_shell_dmax(const double* D, int n, const int* sh_f0, const int* sh_nc, int nsh, float* dsh,
+                                 hipStream_t s) {
+    const size_t tot = (size_t)nsh * nsh;
+    unsigned nb = (unsigned)((tot + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(shell_dmax_kernel, dim3(nb), dim3(256), 0, s, D, n, sh_f0, sh_nc, nsh, dsh);
     return hipGetLastError();
 }
 

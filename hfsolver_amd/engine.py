@@ -49,6 +49,7 @@ SIGNATURES = {
     "hfx_int2": (_I, [_P, _P, ctypes.c_longlong]),
     "hfx_shells": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
     "hfx_screening": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "hfx_set_density_screening": (_I, [_P, _D]),
     "hfx_pair_bounds": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
     "hfx_last_error": (ctypes.c_char_p, [_P]),
     "hfx_destroy": (None, [_P]),
@@ -220,6 +221,11 @@ class Engine:
         Q = np.zeros(npr.value)
         self._check(lib().hfx_pair_bounds(self._h, ctypes.byref(npr), _ptr(A), _ptr(B), _ptr(Q)))
         return A, B, Q
+
+    def set_density_screening(self, threshold):
+        """Skip quartets whose Schwarz bound times the largest density factor is
+        below threshold in the following Fock builds (0: off)."""
+        self._check(lib().hfx_set_density_screening(self._h, float(threshold)))
 
     def screening(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -68,11 +68,20 @@ def _diis_extrapolate(Fs, errs):
     return sum(ci * Fi for ci, Fi in zip(c, Fs))
 
 
-def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device):
+def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device,
+                  incremental=0.0, rebuild=0):
     """doscf with the linear algebra on the GPU (torch on ROCm: rocBLAS / rocSOLVER
     in FP64) and the density kept in HBM (engine.fock_2e_device). F C = S C lam is
     solved as (L^-1 F L^-T) Y = Y lam, C = L^-T Y, with S = L L^T (Cholesky) -- the
-    reduction dsygvd performs for the lower triangle (src/linalg.f90:241-280)."""
+    reduction dsygvd performs for the lower triangle (src/linalg.f90:241-280).
+
+    incremental > 0: G(P) = G(P_prev) + G(P - P_prev), the difference built with
+    density-weighted screening at that threshold (quartets with Q_ab Q_cd
+    max|dP| below it are skipped); a full build every `rebuild` iterations when
+    rebuild > 0 (such a build moves the energy by the accumulated screening
+    error at once, which can keep a tight energy test from settling). At
+    convergence E0, F and G are recomputed from one unscreened build of the
+    final input density."""
     import torch
     dev = torch.device("cuda", device)
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
@@ -83,66 +92,103 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
     P = torch.zeros((n, n), dtype=torch.float64, device=dev) if P0 is None else t(P0)
     G = torch.empty((n, n), dtype=torch.float64, device=dev)
     Escf, Fs, errs = [], [], []
-    for it in range(1, Nscf + 1):
-        engine.fock_2e_device(P, G)
-        F = Ht + G
-        E0 = 0.5 * float(torch.sum(P * (Ht + F)))
-        Escf.append(E0)
-        if callback is not None:
-            callback(it, E0, P, G)
-        Fd = F
-        if diis and bool(torch.any(P != 0)):
-            err = F @ P @ St - St @ P @ F
-            Fs.append(F)
-            errs.append(err)
-            if len(Fs) > diis_size:
-                Fs.pop(0)
-                errs.pop(0)
-            if len(Fs) >= 2:
-                m = len(Fs)
-                B = -np.ones((m + 1, m + 1))
-                B[m, m] = 0.0
-                for i in range(m):
-                    for j in range(m):
-                        B[i, j] = float(torch.sum(errs[i] * errs[j]))
-                rhs = np.zeros(m + 1)
-                rhs[m] = -1.0
-                try:
-                    c = np.linalg.solve(B, rhs)[:m]
-                    Fd = sum(float(ci) * Fi for ci, Fi in zip(c, Fs))
-                except np.linalg.LinAlgError:
-                    pass
-        A = Li @ Fd @ Li.T
-        lam, Y = torch.linalg.eigh(0.5 * (A + A.T))
-        C = Li.T @ Y
-        Pold = P
-        Co = C[:, :nocc]
-        P = 2.0 * Co @ Co.T
-        dP = math.sqrt(float(torch.sum((P - Pold) ** 2)) / 4)
-        if it > 3:
-            w = Escf[it - 4:it]
-            if dP < tolP and max(w) - min(w) < tolE:
-                cpu = lambda x: x.cpu().numpy()
-                return dict(C=cpu(C), P=cpu(P), Pin=cpu(Pold), lam=cpu(lam), E0=E0, F=cpu(F), G=cpu(G),
-                            iterations=it, dP=dP, E_iter=Escf)
-        if not diis:
-            P = (1 - alpha) * Pold + alpha * P
+    Pb, Gb = None, None   # density and G of the last build (incremental mode)
+    if incremental > 0:
+        engine.set_density_screening(incremental)
+    try:
+        for it in range(1, Nscf + 1):
+            if incremental > 0 and Pb is not None and (rebuild <= 0 or (it - 1) % rebuild != 0):
+                dG = torch.empty_like(G)
+                engine.fock_2e_device((P - Pb).contiguous(), dG)
+                G = Gb + dG
+            else:
+                G = torch.empty_like(G)
+                engine.fock_2e_device(P, G)
+            Pb, Gb = P, G
+            res = _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP)
+            if isinstance(res, dict):
+                if incremental > 0:
+                    engine.set_density_screening(0.0)
+                    Pin = t(res["Pin"])
+                    Gx = torch.empty_like(G)
+                    engine.fock_2e_device(Pin, Gx)
+                    F = Ht + Gx
+                    res.update(E0=0.5 * float(torch.sum(Pin * (Ht + F))), F=F.cpu().numpy(), G=Gx.cpu().numpy())
+                return res
+            P = res
+    finally:
+        if incremental > 0:
+            engine.set_density_screening(0.0)
     raise RuntimeError("SCF did not converge in %d iterations." % Nscf)
 
 
+def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP):
+    """One iteration of _doscf_device after the Fock build: the converged result
+    (dict) or the next density."""
+    import torch
+    F = Ht + G
+    E0 = 0.5 * float(torch.sum(P * (Ht + F)))
+    Escf.append(E0)
+    if callback is not None:
+        callback(it, E0, P, G)
+    Fd = F
+    if diis and bool(torch.any(P != 0)):
+        err = F @ P @ St - St @ P @ F
+        Fs.append(F)
+        errs.append(err)
+        if len(Fs) > diis_size:
+            Fs.pop(0)
+            errs.pop(0)
+        if len(Fs) >= 2:
+            m = len(Fs)
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for i in range(m):
+                for j in range(m):
+                    B[i, j] = float(torch.sum(errs[i] * errs[j]))
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            try:
+                c = np.linalg.solve(B, rhs)[:m]
+                Fd = sum(float(ci) * Fi for ci, Fi in zip(c, Fs))
+            except np.linalg.LinAlgError:
+                pass
+    A = Li @ Fd @ Li.T
+    lam, Y = torch.linalg.eigh(0.5 * (A + A.T))
+    C = Li.T @ Y
+    Pold = P
+    Co = C[:, :nocc]
+    P = 2.0 * Co @ Co.T
+    dP = math.sqrt(float(torch.sum((P - Pold) ** 2)) / 4)
+    if it > 3:
+        w = Escf[it - 4:it]
+        if dP < tolP and max(w) - min(w) < tolE:
+            cpu = lambda x: x.cpu().numpy()
+            return dict(C=cpu(C), P=cpu(P), Pin=cpu(Pold), lam=cpu(lam), E0=E0, F=cpu(F), G=cpu(G),
+                        iterations=it, dP=dP, E_iter=Escf)
+    if not diis:
+        P = (1 - alpha) * Pold + alpha * P
+    return P
+
+
 def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=False, diis_size=8, P0=None,
-          callback=None, eig="host", engine=None, device=0):
+          callback=None, eig="host", engine=None, device=0, incremental=0.0):
     """The SCF loop of src/scf.f90:72-136 with G(P) from `fock_2e`.
 
     eig="host": LAPACK dsygvd on the lower triangle as the reference does;
     eig="device": the same steps on the GPU (needs `engine`, whose
-    fock_2e_device is used instead of fock_2e)."""
+    fock_2e_device is used instead of fock_2e); there incremental > 0 builds
+    G(P) = G(P_prev) + G(P - P_prev) with density-weighted screening at that
+    threshold (no counterpart in the reference)."""
     if eig == "device":
         if engine is None:
             raise ValueError("doscf(eig='device') needs the engine")
-        return _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device)
+        return _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device,
+                             incremental)
     if eig != "host":
         raise ValueError("eig must be 'host' or 'device'")
+    if incremental:
+        raise ValueError("incremental Fock builds need eig='device'")
     n = H.shape[0]
     P = np.zeros((n, n)) if P0 is None else np.array(P0, dtype=float)
     Escf, Fs, errs = [], [], []
@@ -178,7 +224,8 @@ def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=F
 
 
 def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=False, P0=None, fp32_threshold=0.0,
-              schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0, int2=False, eig="host"):
+              schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0, int2=False, eig="host",
+              incremental=0.0):
     """Closed-shell RHF of a neutral molecule (src/drivers.f90:12-51).
 
     With int2=True the result also carries `int2`, every unique (ij|kl) in the
@@ -192,7 +239,7 @@ def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=Fals
         H = T + V
         nelec = int(np.sum(atno))
         r = doscf(H, S, nelec // 2, lambda P: eng.fock_2e(P)[0], Nscf, alpha, tolE, tolP, diis=diis, P0=P0,
-                  eig=eig, engine=eng, device=device)
+                  eig=eig, engine=eng, device=device, incremental=incremental)
     except Exception:
         if engine is None and not keep_engine:
             eng.close()

@@ -149,6 +149,15 @@ int hfx_shells(const hfx_t *h, int *nshell, int *first_bf, int *L, int *atom);
  * (arrays of nshell*(nshell+1)/2, may be NULL) receive the shells and bounds. */
 int hfx_pair_bounds(const hfx_t *h, int *npairs, int *A, int *B, double *Q);
 
+/* Density-weighted screening of the following Fock builds (no counterpart in
+ * the reference, whose formG contracts every stored integral; used by
+ * incremental builds G(P) = G(P_prev) + G(P - P_prev) in the SCF): a screened
+ * quartet is skipped when Q_ab Q_cd times the largest |D| over its six density
+ * blocks (ab, cd, ac, ad, bc, bd; D the symmetrised density, per shell-pair
+ * block) is below threshold. 0 (the default) turns it off. Applies to the
+ * row-mode launches (all classes at the benchmark sizes). */
+int hfx_set_density_screening(hfx_t *h, double threshold);
+
 /* Statistics of the last set-up: number of significant shell pairs and of
  * screened shell quartets in FP64 / FP32. */
 int hfx_screening(const hfx_t *h, double *npairs, double *nq_fp64, double *nq_fp32);
