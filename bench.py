@@ -106,6 +106,8 @@ def main():
                     help="also run N consecutive energy+force evaluations (SCF warm-started from the previous "
                          "density) on seeded +-0.01 A perturbations, mixed engine vs FP64 engine (10 in the "
                          "mixed-precision force workload); reported under 'forces'")
+    ap.add_argument("--incremental", type=float, default=0.0, metavar="THR",
+                    help="force workload: incremental Fock builds with density-weighted screening at THR (0: off)")
     ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known bytes) to scale HBM counters")
@@ -229,7 +231,7 @@ def main():
         from hfsolver_amd.clusters import ANG2BOHR
         from hfsolver_amd.grad import ForceField
         rng = np.random.Generator(np.random.PCG64(args.seed + 1))
-        kw = dict(Nscf=80, tolE=1e-10, tolP=1e-7, diis=True, eig="device")
+        kw = dict(Nscf=80, tolE=1e-10, tolP=1e-7, diis=True, eig="device", incremental=args.incremental)
         ffm = ForceField(atno, xyz, fp32_threshold=args.fp32_threshold, **kw)
         ff64 = ForceField(atno, xyz, fp32_threshold=0.0, **kw)
         if r64 is not None:   # the first evaluation starts from the base geometry's density
@@ -257,7 +259,8 @@ def main():
                   "scf_iterations_mixed": its, "force_error_max_vs_fp64": max(gmax),
                   "force_error_rms_vs_fp64": float(np.sqrt(np.mean(np.square(grms)))),
                   "abs_dE_max_vs_fp64": max(dEs),
-                  "perturbation": "uniform +-0.01 A per coordinate about the base geometry, seed %d" % (args.seed + 1)}
+                  "perturbation": "uniform +-0.01 A per coordinate about the base geometry, seed %d" % (args.seed + 1),
+                  "incremental_threshold": args.incremental}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(e64, b, st64[0].quartets_fp64)
 
