@@ -93,6 +93,7 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
     G = torch.empty((n, n), dtype=torch.float64, device=dev)
     Escf, Fs, errs = [], [], []
     Pb, Gb = None, None   # density and G of the last build (incremental mode)
+    dstate = {"B": np.zeros((0, 0))}
     if incremental > 0:
         engine.set_density_screening(incremental)
     try:
@@ -105,7 +106,8 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
                 G = torch.empty_like(G)
                 engine.fock_2e_device(P, G)
             Pb, Gb = P, G
-            res = _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP)
+            res = _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP,
+                               dstate)
             if isinstance(res, dict):
                 if incremental > 0:
                     engine.set_density_screening(0.0)
@@ -122,9 +124,10 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
     raise RuntimeError("SCF did not converge in %d iterations." % Nscf)
 
 
-def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP):
+def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP, dstate):
     """One iteration of _doscf_device after the Fock build: the converged result
-    (dict) or the next density."""
+    (dict) or the next density. dstate["B"] keeps the inner products of the
+    DIIS error vectors, so that each iteration adds one row (one transfer)."""
     import torch
     F = Ht + G
     E0 = 0.5 * float(torch.sum(P * (Ht + F)))
@@ -136,16 +139,23 @@ def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, ca
         err = F @ P @ St - St @ P @ F
         Fs.append(F)
         errs.append(err)
+        row = torch.stack([torch.sum(e * err) for e in errs]).cpu().numpy()
+        Bo = dstate["B"]
+        m = len(errs)
+        Bn = np.zeros((m, m))
+        Bn[:m - 1, :m - 1] = Bo
+        Bn[m - 1, :] = row
+        Bn[:, m - 1] = row
         if len(Fs) > diis_size:
             Fs.pop(0)
             errs.pop(0)
+            Bn = Bn[1:, 1:]
+        dstate["B"] = Bn
         if len(Fs) >= 2:
             m = len(Fs)
             B = -np.ones((m + 1, m + 1))
             B[m, m] = 0.0
-            for i in range(m):
-                for j in range(m):
-                    B[i, j] = float(torch.sum(errs[i] * errs[j]))
+            B[:m, :m] = Bn
             rhs = np.zeros(m + 1)
             rhs[m] = -1.0
             try:
