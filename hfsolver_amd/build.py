@@ -51,7 +51,7 @@ def build_library(verbose=True):
         list(ex.map(lambda j: run([HIPCC] + j), jobs))
     tmp = LIB + ".tmp"
     run([HIPCC, "-shared", "-fopenmp", "--offload-arch=" + ARCH, "-o", tmp] + objs +
-        ["-Wl,-rpath,/opt/rocm/lib:/opt/rocm/llvm/lib"])
+        ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib:/opt/rocm/llvm/lib"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -20,6 +20,7 @@
 // gradient, the two-electron gradient and the store mode run on the GPU too.
 #include <hip/hip_runtime.h>
 #include <omp.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <chrono>
@@ -121,6 +122,7 @@ struct hfx_engine {
     int row_rmax = HFX_ROWS_MAX;
     size_t row_pair_bytes = 32 * 1024;
     double row_blocks = 128;   // blocks a launch is cut into when its rows are few (HFX_ROW_BLOCKS)
+    rocblas_handle blas = nullptr;   // created on first use (AO -> MO transform)
     // density-weighted screening of the Fock builds (hfx_set_density_screening)
     double dthr = 0.0;
     float* d_dsh = nullptr;
@@ -1354,6 +1356,65 @@ int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
     return 0;
 }
 
+int hfx_transform_int2(hfx_t* h, const double* int2, const double* C, double* moint2, long long size) {
+    if (!h || !int2 || !C || !moint2) return h ? fail(h, "hfx_transform_int2: null argument") : 1;
+    const long long n = h->nbf, m = n * (n + 1) / 2;
+    if (size != m * (m + 1) / 2)
+        return fail(h, "hfx_transform_int2: size %lld, expected %lld for %lld functions", size, m * (m + 1) / 2, n);
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n4 = (size_t)n * n * n * n;
+    size_t freeb = 0, totb = 0;
+    HIPCHK(h, hipMemGetInfo(&freeb, &totb));
+    const size_t need = 2 * n4 * sizeof(double) + (size_t)size * sizeof(double) + (size_t)n * n * sizeof(double);
+    if (need + (256u << 20) > freeb)
+        return fail(h, "hfx_transform_int2: needs %.1f GB of device memory, %.1f GB free", need * 1e-9, freeb * 1e-9);
+    if (!h->blas) {
+        if (rocblas_create_handle(&h->blas) != rocblas_status_success) return fail(h, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(h->blas, h->stream) != rocblas_status_success) return fail(h, "rocblas_set_stream failed");
+    struct Scoped {   // device buffers of this call, freed on every return path
+        void* p = nullptr;
+        ~Scoped() {
+            if (p) (void)hipFree(p);
+        }
+    } bX, bY, bP, bC;
+    if (dmalloc(h, &bX.p, n4 * sizeof(double), false) || dmalloc(h, &bY.p, n4 * sizeof(double), false) ||
+        dmalloc(h, &bP.p, (size_t)size * sizeof(double), false) ||
+        dmalloc(h, &bC.p, (size_t)n * n * sizeof(double), false))
+        return 1;
+    double *X = (double*)bX.p, *Y = (double*)bY.p, *P = (double*)bP.p, *Cd = (double*)bC.p;
+    HIPCHK(h, hipMemcpyAsync(P, int2, (size_t)size * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(Cd, C, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hfx_launch_unpack_int2(P, X, (int)n, h->stream));
+    // four quarter transforms as column-major GEMMs (C(mu, i) column-major, the
+    // array [a][b][c][d] seen as a matrix whose rows run over d, then (c, d), ...):
+    const double one = 1.0, zero = 0.0;
+    const rocblas_int in = (rocblas_int)n, in2 = (rocblas_int)(n * n), in3 = (rocblas_int)(n * n * n);
+    rocblas_status st;
+    //  d -> l: Y[l, abc] = sum_d C[d, l] X[d, abc]          -> [a][b][c][l]
+    st = rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, in, in3, in, &one, Cd, in, X, in,
+                       &zero, Y, in);
+    //  c -> k: per (a, b): X_ab[l, k] = sum_c Y_ab[l, c] C[c, k] -> [a][b][k][l]
+    if (st == rocblas_status_success)
+        st = rocblas_dgemm_strided_batched(h->blas, rocblas_operation_none, rocblas_operation_none, in, in, in, &one, Y,
+                                           in, (rocblas_stride)n * n, Cd, in, 0, &zero, X, in, (rocblas_stride)n * n,
+                                           in2);
+    //  b -> j: per a: Y_a[(k, l), j] = sum_b X_a[(k, l), b] C[b, j] -> [a][j][k][l]
+    if (st == rocblas_status_success)
+        st = rocblas_dgemm_strided_batched(h->blas, rocblas_operation_none, rocblas_operation_none, in2, in, in, &one, X,
+                                           in2, (rocblas_stride)n * n * n, Cd, in, 0, &zero, Y, in2,
+                                           (rocblas_stride)n * n * n, in);
+    //  a -> i: X[(j, k, l), i] = sum_a Y[(j, k, l), a] C[a, i] -> [i][j][k][l]
+    if (st == rocblas_status_success)
+        st = rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, in3, in, in, &one, Y, in3, Cd, in,
+                           &zero, X, in3);
+    if (st != rocblas_status_success) return fail(h, "hfx_transform_int2: rocBLAS error %d", (int)st);
+    HIPCHK(h, hfx_launch_pack_int2(X, P, (int)n, h->stream));
+    HIPCHK(h, hipMemcpyAsync(moint2, P, (size_t)size * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 int hfx_int2(hfx_t* h, double* int2, long long size) {
     if (!h || !int2) return h ? fail(h, "hfx_int2: null output") : 1;
     const long long n = h->nbf, m = n * (n + 1) / 2, need = m * (m + 1) / 2;
@@ -1492,6 +1553,7 @@ void hfx_destroy(hfx_t* h) {
         if (h->evaux[i]) (void)hipEventDestroy(h->evaux[i]);
     for (int i = 0; i < HFX_AUX_STREAMS; i++)
         if (h->aux[i]) (void)hipStreamDestroy(h->aux[i]);
+    if (h->blas) (void)rocblas_destroy_handle(h->blas);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

@@ -253,6 +253,9 @@ hipError_t hfx_launch_ket_gather(const KetTab& kt, int ncomp, int nd, const doub
 hipError_t hfx_launch_ket_scatter(const KetTab& kt, int ncomp, int nd, double* Jb, int n, hipStream_t s);
 hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* scale, double* G, double* J, double* K,
                              int n, hipStream_t s);
+// X (n^4, [a][b][c][d]) <- packed 8-fold int2; packed <- canonical elements of Y
+hipError_t hfx_launch_unpack_int2(const double* int2, double* X, int n, hipStream_t s);
+hipError_t hfx_launch_pack_int2(const double* Y, double* out, int n, hipStream_t s);
 // dsh[i * nsh + j] = max |D| over the block of shells i, j (rounded up to float)
 hipError_t hfx_launch_shell_dmax(const double* D, int n, const int* sh_f0, const int* sh_nc, int nsh, float* dsh,
                                  hipStream_t s);

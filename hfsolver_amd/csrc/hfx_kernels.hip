@@ -1010,6 +1010,45 @@ hipError_t hfx_launch_ket_scatter(const KetTab& kt, int ncomp, int nd, double* J
     return hipGetLastError();
 }
 
+// 0-based 8-fold packed index of (i j|k l) (the index of cints.c:327-347, 64-bit).
+__device__ HFX_INLINE long long packed_ijkl(long long i, long long j, long long k, long long l) {
+    const long long ij = i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+    const long long kl = k >= l ? k * (k + 1) / 2 + l : l * (l + 1) / 2 + k;
+    return ij >= kl ? ij * (ij + 1) / 2 + kl : kl * (kl + 1) / 2 + ij;
+}
+
+// X[((a n + b) n + c) n + d] = (a b|c d) from the packed vector.
+__global__ __launch_bounds__(256) void unpack_int2_kernel(const double* __restrict__ int2, double* __restrict__ X,
+                                                          int n) {
+    const size_t n1 = n, tot = n1 * n1 * n1 * n1;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const size_t d = k % n1, c = (k / n1) % n1, b = (k / (n1 * n1)) % n1, a = k / (n1 * n1 * n1);
+        X[k] = int2[packed_ijkl(a, b, c, d)];
+    }
+}
+
+// out[packed(i j|k l)] = Y[((i n + j) n + k) n + l] for the canonical i >= j, k >= l, ij >= kl.
+__global__ __launch_bounds__(256) void pack_int2_kernel(const double* __restrict__ Y, double* __restrict__ out, int n) {
+    const size_t n1 = n, tot = n1 * n1 * n1 * n1;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < tot; k += (size_t)gridDim.x * blockDim.x) {
+        const long long l = k % n1, kk = (k / n1) % n1, j = (k / (n1 * n1)) % n1, i = k / (n1 * n1 * n1);
+        if (i < j || kk < l) continue;
+        const long long ij = i * (i + 1) / 2 + j, kl = kk * (kk + 1) / 2 + l;
+        if (ij < kl) continue;
+        out[ij * (ij + 1) / 2 + kl] = Y[k];
+    }
+}
+
+hipError_t hfx_launch_unpack_int2(const double* int2, double* X, int n, hipStream_t s) {
+    hipLaunchKernelGGL(unpack_int2_kernel, dim3(4096), dim3(256), 0, s, int2, X, n);
+    return hipGetLastError();
+}
+
+hipError_t hfx_launch_pack_int2(const double* Y, double* out, int n, hipStream_t s) {
+    hipLaunchKernelGGL(pack_int2_kernel, dim3(4096), dim3(256), 0, s, Y, out, n);
+    return hipGetLastError();
+}
+
 // One thread per shell pair (i, j): the largest |D| of the block, rounded up to
 // float so that the screening test never drops a quartet it should keep.
 __global__ __launch_bounds__(256) void shell_dmax_kernel(const double* __restrict__ D, int n,

@@ -47,6 +47,7 @@ SIGNATURES = {
     "hfx_set_geometry": (_I, [_P, _P]),
     "hfx_eri_shells": (_I, [_P, _I, _P, _P]),
     "hfx_int2": (_I, [_P, _P, ctypes.c_longlong]),
+    "hfx_transform_int2": (_I, [_P, _P, _P, _P, ctypes.c_longlong]),
     "hfx_shells": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
     "hfx_screening": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "hfx_set_density_screening": (_I, [_P, _D]),
@@ -201,6 +202,15 @@ class Engine:
         m = n * (n + 1) // 2
         out = np.zeros(m * (m + 1) // 2)
         self._check(lib().hfx_int2(self._h, _ptr(out), out.size))
+        return out
+
+    def transform_int2(self, int2, C):
+        """Orbital integrals (i j|k l) in the packed order of int2 (transform_int2
+        of mbpt.f90, four GPU quarter transforms); C[:, i] = orbital i."""
+        int2 = _f64(int2)
+        C = np.asfortranarray(C, dtype=np.float64)
+        out = np.zeros_like(int2)
+        self._check(lib().hfx_transform_int2(self._h, _ptr(int2), _ptr(C), _ptr(out), out.size))
         return out
 
     def shells(self):

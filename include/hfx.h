@@ -149,6 +149,16 @@ int hfx_shells(const hfx_t *h, int *nshell, int *first_bf, int *L, int *atom);
  * (arrays of nshell*(nshell+1)/2, may be NULL) receive the shells and bounds. */
 int hfx_pair_bounds(const hfx_t *h, int *npairs, int *A, int *B, double *Q);
 
+/*
+ * Orbital integrals from the packed AO integrals of hfx_int2: the four-index
+ * transformation of transform_int2 (src/mbpt.f90:11-79), on the GPU:
+ *   moint2[index(i,j,k,l)] = sum C(mu,i) C(nu,j) C(al,k) C(be,l) int2[index(mu,nu,al,be)],
+ * C: n x n column-major (column i = orbital i), int2 / moint2: the packed
+ * 8-fold vectors of `size` = m(m+1)/2 elements, m = n(n+1)/2. Needs
+ * 2 n^4 doubles of device memory (checked; an error names the shortfall).
+ */
+int hfx_transform_int2(hfx_t *h, const double *int2, const double *C, double *moint2, long long size);
+
 /* Density-weighted screening of the following Fock builds (no counterpart in
  * the reference, whose formG contracts every stored integral; used by
  * incremental builds G(P) = G(P_prev) + G(P - P_prev) in the SCF): a screened

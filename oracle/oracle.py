@@ -15,6 +15,8 @@ hfsolver's src/):
   gaussints         basis.f90:490-569   (S, T, V, optional packed int2)
   doscf             scf.f90:72-136      (P=0 start, formP, E0, convergence, mixing)
   rhf_gauss         drivers.f90:12-51
+  transform_int2    mbpt.f90:11-79      (AO -> MO packed integrals)
+  mbpt2             mbpt.f90:154-159 + mbpt2.f90:8-31 (closed-shell second order)
 The C half (hfx_oracle.c, loaded through ctypes) holds the Boys function and
 the THO integrals.
 
@@ -231,6 +233,64 @@ def intindex(i, j, k, l):
     if ij < kl:
         ij, kl = kl, ij
     return ij * (ij + 1) // 2 + kl
+
+
+def unpack_int2(int2, n):
+    """Full (i j|k l) array [n, n, n, n] from the packed 8-fold int2 (0-based
+    index of cints.c:327-347)."""
+    i = np.arange(n)
+    I, J = np.meshgrid(i, i, indexing="ij")
+    ij = np.where(I >= J, I * (I + 1) // 2 + J, J * (J + 1) // 2 + I)
+    a = ij[:, :, None, None]
+    b = ij[None, None, :, :]
+    hi = np.maximum(a, b)
+    lo = np.minimum(a, b)
+    return int2[hi * (hi + 1) // 2 + lo]
+
+
+def pack_int2(Y):
+    """Packed 8-fold vector of a full [n, n, n, n] array with that symmetry."""
+    n = Y.shape[0]
+    m = n * (n + 1) // 2
+    out = np.zeros(m * (m + 1) // 2)
+    i = np.arange(n)
+    I, J = np.meshgrid(i, i, indexing="ij")
+    keep = I >= J
+    ij = (I * (I + 1) // 2 + J)[keep]
+    pairs = np.stack([I[keep], J[keep]], axis=1)
+    a = ij[:, None]
+    b = ij[None, :]
+    sel = a >= b
+    idx = (a * (a + 1) // 2 + b)[sel]
+    pi = np.broadcast_to(np.arange(len(ij))[:, None], sel.shape)[sel]
+    pk = np.broadcast_to(np.arange(len(ij))[None, :], sel.shape)[sel]
+    out[idx] = Y[pairs[pi, 0], pairs[pi, 1], pairs[pk, 0], pairs[pk, 1]]
+    return out
+
+
+def transform_int2(int2, C):
+    """mbpt.f90:11-79: orbital integrals (i j|k l) = sum C_mu,i C_nu,j C_al,k C_be,l
+    (mu nu|al be), in the packed order of the input."""
+    n = C.shape[0]
+    T = unpack_int2(int2, n)
+    Y = np.einsum("abcd,ai,bj,ck,dl->ijkl", T, C, C, C, C, optimize=True)
+    return pack_int2(Y)
+
+
+def mbpt2(moint2, lam, nocc):
+    """Second-order energy, mbpt.f90:154-159 with mbpt2.f90:8-31:
+    sum over occupied a, b and virtual r, s of
+    [2 (ar|bs)(ra|sb) - (ar|bs)(sa|rb)] / (lam_a + lam_b - lam_r - lam_s)."""
+    n = len(lam)
+    Y = unpack_int2(moint2, n)
+    o, v = slice(0, nocc), slice(nocc, n)
+    vovo = Y[v, o, v, o]
+    arbs = Y[o, v, o, v]                   # (a r|b s) at [a, r, b, s]
+    rasb = vovo.transpose(1, 0, 3, 2)      # (r a|s b) at [a, r, b, s]
+    sarb = vovo.transpose(1, 2, 3, 0)      # (s a|r b) at [a, r, b, s]
+    lo, lv = lam[o], lam[v]
+    den = lo[:, None, None, None] - lv[None, :, None, None] + lo[None, None, :, None] - lv[None, None, None, :]
+    return float(np.sum((2.0 * arbs * rasb - arbs * sarb) / den))
 
 
 def eri(B, i, j, k, l):

@@ -77,3 +77,23 @@ def test_reference_molecule_tests_with_gpu_getints2(tmp_path):
     say(tail)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     assert "E2    (MBPT2)" in r.stdout
+
+
+def test_transform_int2_matches_oracle_and_mbpt2_known_answer(golden):
+    """GPU four-index transformation (hfx_transform_int2) against the oracle's
+    restatement of transform_int2 on CH4, and the reference's own second-order
+    energy of CH4 (test_gaussians_molecules.f90:114, 1e-7) from its output."""
+    g = golden("ch4")
+    say("AO -> MO transform, CH4")
+    r = O.rhf_gauss(g["atno"], g["xyz"])
+    eng = Engine(Basis(g["atno"], g["xyz"]), schwarz_threshold=0.0)
+    int2 = eng.int2()
+    mo = eng.transform_int2(int2, r.C)
+    ref = O.transform_int2(int2, r.C)
+    say("  max |moint2 - oracle| %.2e" % np.abs(mo - ref).max())
+    assert np.abs(mo - ref).max() < 1e-12
+    E2 = O.mbpt2(mo, r.lam, int(sum(g["atno"])) // 2)
+    assert abs(E2 - (-0.16815509)) < 1e-7
+    with pytest.raises(Exception):
+        eng.transform_int2(int2[:-1], r.C)
+    eng.close()

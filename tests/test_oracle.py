@@ -148,3 +148,22 @@ def test_gradient_finite_differences():
             e.append(O.rhf_gauss(atno, x, tolP=1e-10, tolE=1e-12, Nscf=80).Etot)
         fd = (e[0] - e[1]) / (2 * h)
         assert abs(fd - g[a, d]) < 1e-6
+
+
+def test_transform_int2_and_mbpt2_known_answers():
+    """The oracle's restatement of transform_int2 (mbpt.f90:11-79) and mbpt2
+    (mbpt.f90:154-159, mbpt2.f90:8-31): the reference's CH4 second-order energy
+    (test_gaussians_molecules.f90:114, 1e-7), the SCF energy rebuilt from the
+    orbital integrals, and the identity transform."""
+    r = O.rhf_gauss(*CH4)
+    B = r.basis
+    int2 = O.get_ints(B)
+    n = B.nbf
+    assert np.array_equal(O.transform_int2(int2, np.eye(n)), int2)
+    mo = O.transform_int2(int2, r.C)
+    E2 = O.mbpt2(mo, r.lam, 5)
+    assert abs(E2 - (-0.16815509)) < 1e-7
+    Y = O.unpack_int2(mo, n)
+    h = r.C.T @ r.H @ r.C
+    E0 = 2 * np.trace(h[:5, :5]) + sum(2 * Y[a, a, b, b] - Y[a, b, b, a] for a in range(5) for b in range(5))
+    assert abs(E0 - r.E0) < 1e-11
