@@ -22,7 +22,10 @@ the THO integrals.
 
 Parity pins: tests/test_oracle.py checks every function above against the
 reference's compiled outputs (tests/golden/*.json), the reference's literature
-energies, closed forms (Boys F0, S_ii = 1) and brute-force sums.
+energies, closed forms (Boys F0, S_ii = 1) and brute-force sums;
+transform_int2 and mbpt2 against the reference's CH4 second-order energy
+(test_gaussians_molecules.f90:114), the SCF energy rebuilt from the orbital
+integrals, and the identity transform.
 """
 import ctypes
 import math
