@@ -15,6 +15,7 @@ hfsolver's src/):
   gaussints         basis.f90:490-569   (S, T, V, optional packed int2)
   doscf             scf.f90:72-136      (P=0 start, formP, E0, convergence, mixing)
   rhf_gauss         drivers.f90:12-51
+  libint_shell_walk gaussians_libint.f90:70-124 (shells from per-function powers)
   transform_int2    mbpt.f90:11-79      (AO -> MO packed integrals)
   mbpt2             mbpt.f90:154-159 + mbpt2.f90:8-31 (closed-shell second order)
 The C half (hfx_oracle.c, loaded through ctypes) holds the Boys function and
@@ -236,6 +237,20 @@ def intindex(i, j, k, l):
     if ij < kl:
         ij, kl = kl, ij
     return ij * (ij + 1) // 2 + kl
+
+
+def libint_shell_walk(power):
+    """Shells as the libint plug-in recovers them from the per-function powers
+    (gaussians_libint.f90:70-124): from function i, lambda = sum(power(:, i)),
+    the next shell starts (lambda+1)(lambda+2)/2 functions later.
+    Returns [(first function, lambda)], 0-based."""
+    power = np.asarray(power)
+    out, i = [], 0
+    while i < power.shape[1]:
+        lam = int(power[:, i].sum())
+        out.append((i, lam))
+        i += (lam + 1) * (lam + 2) // 2
+    return out
 
 
 def unpack_int2(int2, n):

@@ -97,3 +97,17 @@ def test_transform_int2_matches_oracle_and_mbpt2_known_answer(golden):
     with pytest.raises(Exception):
         eng.transform_int2(int2[:-1], r.C)
     eng.close()
+
+
+@pytest.mark.parametrize("name", ["h2o_config0", "ch4", "water2_seed2"])
+def test_shells_match_the_libint_plugin_walk(golden, name):
+    """The engine regroups the per-function arrays into shells exactly as the
+    libint plug-in walks them (gaussians_libint.f90:70-124), the other
+    implementation of the getints2 contract."""
+    g = golden(name)
+    b = Basis(g["atno"], g["xyz"])
+    eng = Engine(b, schwarz_threshold=0.0)
+    ns, f0, Ls, _ = eng.shells()
+    eng.close()
+    walk = O.libint_shell_walk(np.stack([b.l, b.m, b.n]))
+    assert sorted(zip(f0.tolist(), Ls.tolist())) == walk

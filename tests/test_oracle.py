@@ -167,3 +167,17 @@ def test_transform_int2_and_mbpt2_known_answers():
     h = r.C.T @ r.H @ r.C
     E0 = 2 * np.trace(h[:5, :5]) + sum(2 * Y[a, a, b, b] - Y[a, b, b, a] for a in range(5) for b in range(5))
     assert abs(E0 - r.E0) < 1e-11
+
+
+def test_libint_shell_walk_counts():
+    """The libint plug-in's shell walk (gaussians_libint.f90:70-124) on one
+    6-31G** water: 12 shells, 7 s + 4 p + 1 d (p631ss.txt:1-9,136-158), which
+    cover the 25 functions in order."""
+    from hfsolver_amd.basis import Basis as HB
+    hb = HB(*H2O_TEST)
+    power = np.stack([hb.l, hb.m, hb.n])
+    walk = O.libint_shell_walk(power)
+    lams = [lam for _, lam in walk]
+    assert len(walk) == 12 and lams.count(0) == 7 and lams.count(1) == 4 and lams.count(2) == 1
+    nxt = [f + (lam + 1) * (lam + 2) // 2 for f, lam in walk]
+    assert [f for f, _ in walk][1:] == nxt[:-1] and nxt[-1] == power.shape[1] == 25
