@@ -236,8 +236,8 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const BP* __restrict_
         const PT kq = PT(Qp.K) * iq;   // the ket's factor K / q of the prefactor
         for (int i1 = 0; i1 < bra.np; i1++) {
             const BP P = hfx_ld_prim(prims + bra.p0 + i1);
-            // primitive-level arithmetic in the kernel's type (FP32 kernels: roots
-            // and weights from the single-precision rules and tables)
+            // primitive-level arithmetic in the kernel's type (the Rys rule of the
+            // FP32 kernels in FP64 unless HFX_FP32_RULES)
             const PT p = PT(P.p), pq = p + q;
             const PT ip = PT(P.ip);
             const PT isq = hfx_rsqrt(pq);
