@@ -78,8 +78,8 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
     incremental > 0: G(P) = G(P_prev) + G(P - P_prev), the difference built with
     density-weighted screening at that threshold (quartets with Q_ab Q_cd
     max|dP| below it are skipped); a full build every `rebuild` iterations when
-    rebuild > 0 (such a build moves the energy by the accumulated screening
-    error at once, which can keep a tight energy test from settling). At
+    rebuild > 0, and full builds once the density change is below 10 tolP, so
+    that the last iterations carry no accumulated screening noise. At
     convergence E0, F and G are recomputed from one unscreened build of the
     final input density."""
     import torch
@@ -98,7 +98,10 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
         engine.set_density_screening(incremental)
     try:
         for it in range(1, Nscf + 1):
-            if incremental > 0 and Pb is not None and (rebuild <= 0 or (it - 1) % rebuild != 0):
+            # difference builds while the density still moves; full builds for the
+            # last iterations, so that the energy test sees no screening noise
+            near = dstate.get("dP", 1.0) < 10 * tolP
+            if incremental > 0 and Pb is not None and not near and (rebuild <= 0 or (it - 1) % rebuild != 0):
                 dG = torch.empty_like(G)
                 engine.fock_2e_device((P - Pb).contiguous(), dG)
                 G = Gb + dG
@@ -170,6 +173,7 @@ def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, ca
     Co = C[:, :nocc]
     P = 2.0 * Co @ Co.T
     dP = math.sqrt(float(torch.sum((P - Pold) ** 2)) / 4)
+    dstate["dP"] = dP
     if it > 3:
         w = Escf[it - 4:it]
         if dP < tolP and max(w) - min(w) < tolE:
