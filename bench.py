@@ -108,6 +108,8 @@ def main():
                          "mixed-precision force workload); reported under 'forces'")
     ap.add_argument("--incremental", type=float, default=0.0, metavar="THR",
                     help="force workload: incremental Fock builds with density-weighted screening at THR (0: off)")
+    ap.add_argument("--purify", action="store_true",
+                    help="force workload: SCF densities by canonical purification instead of eigenvectors")
     ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known bytes) to scale HBM counters")
@@ -231,7 +233,8 @@ def main():
         from hfsolver_amd.clusters import ANG2BOHR
         from hfsolver_amd.grad import ForceField
         rng = np.random.Generator(np.random.PCG64(args.seed + 1))
-        kw = dict(Nscf=80, tolE=1e-10, tolP=1e-7, diis=True, eig="device", incremental=args.incremental)
+        kw = dict(Nscf=80, tolE=1e-10, tolP=1e-7, diis=True, eig="device", incremental=args.incremental,
+                  purify=args.purify)
         ffm = ForceField(atno, xyz, fp32_threshold=args.fp32_threshold, **kw)
         ff64 = ForceField(atno, xyz, fp32_threshold=0.0, **kw)
         if r64 is not None:   # the first evaluation starts from the base geometry's density
@@ -260,7 +263,7 @@ def main():
                   "force_error_rms_vs_fp64": float(np.sqrt(np.mean(np.square(grms)))),
                   "abs_dE_max_vs_fp64": max(dEs),
                   "perturbation": "uniform +-0.01 A per coordinate about the base geometry, seed %d" % (args.seed + 1),
-                  "incremental_threshold": args.incremental}
+                  "incremental_threshold": args.incremental, "purification": bool(args.purify)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(e64, b, st64[0].quartets_fp64)
 

@@ -37,10 +37,11 @@ class ForceField:
     from the previous density (the engine is re-used through hfx_set_geometry)."""
 
     def __init__(self, atno, xyz, fp32_threshold=0.0, schwarz_threshold=1e-12, Nscf=60, tolE=1e-10, tolP=1e-8,
-                 diis=True, alpha=1.0, eig="host", incremental=0.0):
+                 diis=True, alpha=1.0, eig="host", incremental=0.0, purify=False):
         self.atno = list(atno)
         self.engine = Engine(Basis(atno, xyz), schwarz_threshold=schwarz_threshold, fp32_threshold=fp32_threshold)
-        self.kw = dict(Nscf=Nscf, tolE=tolE, tolP=tolP, diis=diis, alpha=alpha, eig=eig, incremental=incremental)
+        self.kw = dict(Nscf=Nscf, tolE=tolE, tolP=tolP, diis=diis, alpha=alpha, eig=eig, incremental=incremental,
+                       purify=purify)
         self.P = None
         self.last = None
 

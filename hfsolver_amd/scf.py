@@ -69,7 +69,7 @@ def _diis_extrapolate(Fs, errs):
 
 
 def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device,
-                  incremental=0.0, rebuild=0):
+                  incremental=0.0, rebuild=0, purify=False):
     """doscf with the linear algebra on the GPU (torch on ROCm: rocBLAS / rocSOLVER
     in FP64) and the density kept in HBM (engine.fock_2e_device). F C = S C lam is
     solved as (L^-1 F L^-T) Y = Y lam, C = L^-T Y, with S = L L^T (Cholesky) -- the
@@ -81,7 +81,11 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
     rebuild > 0, and full builds once the density change is below 10 tolP, so
     that the last iterations carry no accumulated screening noise. At
     convergence E0, F and G are recomputed from one unscreened build of the
-    final input density."""
+    final input density.
+
+    purify: the density of each iteration by canonical purification of the
+    orthogonalised Fock matrix (matrix products) instead of its eigenvectors;
+    one eigendecomposition at convergence gives C and lam."""
     import torch
     dev = torch.device("cuda", device)
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
@@ -93,7 +97,7 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
     G = torch.empty((n, n), dtype=torch.float64, device=dev)
     Escf, Fs, errs = [], [], []
     Pb, Gb = None, None   # density and G of the last build (incremental mode)
-    dstate = {"B": np.zeros((0, 0))}
+    dstate = {"B": np.zeros((0, 0)), "purify": purify}
     if incremental > 0:
         engine.set_density_screening(incremental)
     try:
@@ -125,6 +129,40 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
         if incremental > 0:
             engine.set_density_screening(0.0)
     raise RuntimeError("SCF did not converge in %d iterations." % Nscf)
+
+
+def canonical_purification(A, nocc, tol=1e-13, maxit=200):
+    """Projector onto the nocc lowest eigenvectors of the symmetric A (orthogonal
+    basis) by canonical purification (Palser and Manolopoulos, Phys. Rev. B 58,
+    12704 (1998)): matrix products only, trace nocc throughout. The start
+    X0 = (lam/n)(mu I - A) + (nocc/n) I maps the Gershgorin interval of A into
+    [0, 1]; each step X <- ((1 - 2c) X + (1 + c) X^2 - X^3)/(1 - c) for c <= 1/2,
+    ((1 + c) X^2 - X^3)/c otherwise (both keep [0, 1] in place and the trace),
+    with c = tr(X^2 - X^3)/tr(X - X^2), until
+    tr(X - X^2) (the idempotency error) is below tol n."""
+    import torch
+    n = A.shape[0]
+    d = torch.diagonal(A)
+    r = torch.sum(torch.abs(A), dim=1) - torch.abs(d)
+    hmax, hmin = float(torch.max(d + r)), float(torch.min(d - r))
+    mu = float(torch.sum(d)) / n
+    lam = min(nocc / (hmax - mu), (n - nocc) / (mu - hmin))
+    X = (-lam / n) * A
+    X.diagonal().add_(lam * mu / n + nocc / n)
+    for _ in range(maxit):
+        X2 = X @ X
+        X3 = X2 @ X
+        t = torch.stack([torch.trace(X - X2), torch.trace(X2 - X3)]).cpu().numpy()
+        if t[0] <= tol * n:
+            break
+        c = t[1] / t[0]
+        if c <= 0.5:
+            X = ((1 - 2 * c) * X + (1 + c) * X2 - X3) / (1 - c)
+        else:
+            X = ((1 + c) * X2 - X3) / c
+    else:
+        raise RuntimeError("canonical purification did not converge")
+    return 0.5 * (X + X.T)
 
 
 def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP, dstate):
@@ -167,16 +205,27 @@ def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, ca
             except np.linalg.LinAlgError:
                 pass
     A = Li @ Fd @ Li.T
-    lam, Y = torch.linalg.eigh(0.5 * (A + A.T))
-    C = Li.T @ Y
+    A = 0.5 * (A + A.T)
     Pold = P
-    Co = C[:, :nocc]
-    P = 2.0 * Co @ Co.T
+    if dstate.get("purify"):
+        X = canonical_purification(A, nocc)
+        P = 2.0 * Li.T @ X @ Li
+        C = lam = None
+    else:
+        lam, Y = torch.linalg.eigh(A)
+        C = Li.T @ Y
+        Co = C[:, :nocc]
+        P = 2.0 * Co @ Co.T
     dP = math.sqrt(float(torch.sum((P - Pold) ** 2)) / 4)
     dstate["dP"] = dP
     if it > 3:
         w = Escf[it - 4:it]
         if dP < tolP and max(w) - min(w) < tolE:
+            if C is None:   # orbitals and orbital energies of the converged operator
+                lam, Y = torch.linalg.eigh(A)
+                C = Li.T @ Y
+                Co = C[:, :nocc]
+                P = 2.0 * Co @ Co.T
             cpu = lambda x: x.cpu().numpy()
             return dict(C=cpu(C), P=cpu(P), Pin=cpu(Pold), lam=cpu(lam), E0=E0, F=cpu(F), G=cpu(G),
                         iterations=it, dP=dP, E_iter=Escf)
@@ -186,7 +235,7 @@ def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, ca
 
 
 def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=False, diis_size=8, P0=None,
-          callback=None, eig="host", engine=None, device=0, incremental=0.0):
+          callback=None, eig="host", engine=None, device=0, incremental=0.0, purify=False):
     """The SCF loop of src/scf.f90:72-136 with G(P) from `fock_2e`.
 
     eig="host": LAPACK dsygvd on the lower triangle as the reference does;
@@ -198,11 +247,11 @@ def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=F
         if engine is None:
             raise ValueError("doscf(eig='device') needs the engine")
         return _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, P0, callback, device,
-                             incremental)
+                             incremental, purify=purify)
     if eig != "host":
         raise ValueError("eig must be 'host' or 'device'")
-    if incremental:
-        raise ValueError("incremental Fock builds need eig='device'")
+    if incremental or purify:
+        raise ValueError("incremental builds and purification need eig='device'")
     n = H.shape[0]
     P = np.zeros((n, n)) if P0 is None else np.array(P0, dtype=float)
     Escf, Fs, errs = [], [], []
@@ -239,7 +288,7 @@ def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=F
 
 def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=False, P0=None, fp32_threshold=0.0,
               schwarz_threshold=1e-12, engine=None, keep_engine=False, device=0, int2=False, eig="host",
-              incremental=0.0):
+              incremental=0.0, purify=False):
     """Closed-shell RHF of a neutral molecule (src/drivers.f90:12-51).
 
     With int2=True the result also carries `int2`, every unique (ij|kl) in the
@@ -253,7 +302,7 @@ def rhf_gauss(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, *, diis=Fals
         H = T + V
         nelec = int(np.sum(atno))
         r = doscf(H, S, nelec // 2, lambda P: eng.fock_2e(P)[0], Nscf, alpha, tolE, tolP, diis=diis, P0=P0,
-                  eig=eig, engine=eng, device=device, incremental=incremental)
+                  eig=eig, engine=eng, device=device, incremental=incremental, purify=purify)
     except Exception:
         if engine is None and not keep_engine:
             eng.close()

@@ -213,3 +213,24 @@ def test_incremental_scf_with_density_screening():
     G = eng.fock_2e(r0["Pin"])[0]
     assert np.abs(G - r0["G"]).max() < 1e-10
     eng.close()
+
+
+def test_scf_with_purification_matches_eigenvectors():
+    """SCF densities by canonical purification (matrix products on the GPU)
+    reach the same energy, density and orbital energies as the eigenvector
+    route."""
+    from hfsolver_amd.scf import doscf
+    atno, xyz = water_cluster(16, seed=16)
+    b = Basis(atno, xyz)
+    eng = Engine(b)
+    S, T, V = eng.one_electron()
+    nocc = int(sum(atno)) // 2
+    kw = dict(Nscf=60, tolE=1e-10, tolP=1e-7, diis=True, eig="device", engine=eng)
+    say("16 waters: SCF, eigenvectors / purification")
+    r0 = doscf(T + V, S, nocc, None, **kw)
+    r1 = doscf(T + V, S, nocc, None, purify=True, **kw)
+    say("  E0 %.12f vs %.12f, iterations %d vs %d" % (r0["E0"], r1["E0"], r0["iterations"], r1["iterations"]))
+    assert abs(r1["E0"] - r0["E0"]) < 1e-9
+    assert np.abs(r1["P"] - r0["P"]).max() < 1e-5
+    assert np.abs(r1["lam"][:nocc + 5] - r0["lam"][:nocc + 5]).max() < 1e-6
+    eng.close()
