@@ -234,3 +234,28 @@ def test_scf_with_purification_matches_eigenvectors():
     assert np.abs(r1["P"] - r0["P"]).max() < 1e-5
     assert np.abs(r1["lam"][:nocc + 5] - r0["lam"][:nocc + 5]).max() < 1e-6
     eng.close()
+
+
+def test_device_entry_waits_for_the_producing_stream():
+    """fock_2e_device reads dP on the engine's own streams: the binding first
+    waits for torch's stream, so a density written there just before the call
+    (behind a queue of long products) is the one used."""
+    import torch
+    atno, xyz = water_cluster(4, seed=4)
+    b = Basis(atno, xyz)
+    eng = Engine(b)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((b.nbf, b.nbf)) * 0.05
+    P = X + X.T
+    G_ref = eng.fock_2e(P)[0]
+    dev = torch.device("cuda", 0)
+    busy = torch.randn(3000, 3000, dtype=torch.float64, device=dev)
+    dP = torch.zeros(b.nbf * b.nbf, dtype=torch.float64, device=dev)
+    dG = torch.zeros_like(dP)
+    for _ in range(10):
+        busy = (busy @ busy) * 1e-4
+    dP.copy_(torch.as_tensor(np.asfortranarray(P).ravel(order="F")))
+    eng.fock_2e_device(dP, dG)
+    G = dG.cpu().numpy().reshape(b.nbf, b.nbf, order="F")
+    assert np.abs(G - G_ref).max() < 1e-12
+    eng.close()
