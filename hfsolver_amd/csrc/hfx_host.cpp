@@ -381,7 +381,8 @@ FlopModel flop_model(int la, int lb, int lc, int ld) {
     const int NA = ncart(la), NB = ncart(lb), NC = ncart(lc), ND = ncart(ld);
     double f = 0;
     f += 30;                                   // P-Q, T, rho, prefactor (2 div, 1 sqrt)
-    f += 2.0 * NR * (3.0 * RYS_DEG + 3.0) + 8; // Clenshaw for u_i, w_i
+    f += 2.0 * NR * (3.0 * 13 + 3.0) + 8;      // u_i, w_i as a degree-13 series (frozen model; the
+                                               // tables' own degree changes nothing here)
     double per_root = 16;                      // B00, B10, B01, C00, C00' (x3)
     double g = 0;
     g += (LAB > 0 ? 1 : 0) + 3.0 * std::max(0, LAB - 1);
