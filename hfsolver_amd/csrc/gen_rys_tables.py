@@ -22,7 +22,7 @@ Method (all published, nothing tabulated by hand):
     rewritten in powers of the interval variable x in [-1, 1] (in 60-digit
     arithmetic, then rounded): the kernels evaluate it by Horner's rule, one
     multiply-add per degree; the Chebyshev coefficients decay fast enough that
-    sum_k |a_k| stays below 1.3 |f(x)| on every interval, so the power form
+    sum_k |a_k| stays below 1.2 |f(x)| on every interval, so the power form
     loses at most a few ulps;
   * for T >= TMAX the weight exp(-T t^2) is negligible at t = 1 and the rule
     becomes half-range Gauss-Hermite: u_i = x_i^2 / T, w_i = W_i / sqrt(T)
@@ -45,10 +45,10 @@ from multiprocessing import Pool
 import mpmath as mp
 
 NMAX = 5          # enough for (dd|dd) energies and their first derivatives (L <= 9)
-H = 0.25          # interval width in T
-NINT = 240        # intervals on [0, TMAX); exp(-T) T^(m-1/2)/Gamma(m+1/2) < 1e-16 at T = 60 for m <= 9
+H = 0.125         # interval width in T
+NINT = 480        # intervals on [0, TMAX); exp(-T) T^(m-1/2)/Gamma(m+1/2) < 1e-16 at T = 60 for m <= 9
 TMAX = H * NINT
-DEG = 8           # degree per interval: the Chebyshev tail beyond it is < 4e-17 of |f| on every interval
+DEG = 7           # degree per interval: the Chebyshev tail beyond it is < 2e-17 of |f| on every interval
 DEGF = 6          # single-precision copy: the series truncated after degree 6
                   # (dropped coefficients sum to < 1.1e-8 of |c_0| on every interval)
 DPS = 60

@@ -2,13 +2,13 @@
 // Piecewise-polynomial Rys nodes/weights on T in [0, 60), Gauss-Hermite beyond.
 #pragma once
 #define RYS_NMAX 5
-#define RYS_NINT 240
-#define RYS_DEG 8
-#define RYS_H 0.25
+#define RYS_NINT 480
+#define RYS_DEG 7
+#define RYS_H 0.125
 #define RYS_TMAX 60
-#define RYS_POLY_SIZE 64800
+#define RYS_POLY_SIZE 115200
 #define RYS_DEGF 6
-#define RYS_CHEBF_SIZE 50400
+#define RYS_CHEBF_SIZE 100800
 #define BOYS_DT 0.125
 #define BOYS_INV_DT 8
 #define BOYS_TMAX 46
