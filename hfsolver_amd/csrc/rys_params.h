@@ -9,13 +9,13 @@
 #define RYS_POLY_SIZE 115200
 #define RYS_DEGF 6
 #define RYS_CHEBF_SIZE 100800
-#define BOYS_DT 0.125
-#define BOYS_INV_DT 8
+#define BOYS_DT 0.03125
+#define BOYS_INV_DT 32
 #define BOYS_TMAX 46
-#define BOYS_K 9
-#define BOYS_NCOL 12
-#define BOYS_NPT 369
-#define BOYS_SIZE 4428
+#define BOYS_K 7
+#define BOYS_NCOL 10
+#define BOYS_NPT 1473
+#define BOYS_SIZE 14730
 #define BOYSF_K 5
 #define BOYSF_NCOL 8
-#define BOYSF_SIZE 2952
+#define BOYSF_SIZE 11784
