@@ -410,7 +410,7 @@ int compute_schwarz(hfx_t* h) {
         int* d_list = nullptr;
         if (upload(h, &d_list, all[c], false)) return 1;
         tmp.push_back(d_list);
-        HIPCHK(h, hfx_launch_schwarz(c, h->d_pairs, h->d_prims, d_list, (int)all[c].size(), d_q2, h->stream));
+        HIPCHK(h, hfx_launch_schwarz(c, h->d_pairs, h->d_prims, d_list, (int)all[c].size(), h->d_scale, d_q2, h->stream));
     }
     std::vector<unsigned long long> q2(np);
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -244,7 +244,7 @@ hipError_t hfx_launch_grad_rows(int cls_bra, int cls_ket, bool fp32, const GradR
 hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
 // Q2[pair] = max over the diagonal integrals (ab|ab) of the pair (squared bound)
 hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,
-                              unsigned long long* q2bits, hipStream_t s);
+                              const double* scale, unsigned long long* q2bits, hipStream_t s);
 hipError_t hfx_launch_prep(const double* P, const double* scale, double* D, double* Jb, double* Kb, int n,
                            hipStream_t s);
 // kt.Dcd <- D blocks of every ket of the table, kt.Jcd <- 0 (ncomp = NC * ND)

@@ -619,10 +619,15 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
 #endif
 
 #if HFX_PART == 6
-// q2bits[pair] = bits of max_{a,b} |(ab|ab)| (non-negative doubles order as integers)
+// q2bits[pair] = bits of max_{a,b} |(ab|ab)| over the basis functions a, b
+// (non-negative doubles order as integers). The kernels evaluate integrals
+// with the shell's coefficients; a function's own integrals carry its scale
+// (scale[f], e.g. sqrt(3) for d_xy against d_xx), so (ab|ab) of the functions
+// is scale_a^2 scale_b^2 times the shell-coefficient value.
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void schwarz_kernel(const ShellPair* pairs, const PrimPair* prims, const int* list,
-                                                      int npairs, unsigned long long* q2bits) {
+                                                      int npairs, const double* scale,
+                                                      unsigned long long* q2bits) {
     constexpr bool SPA = split_a(LA, LB, LA, LB), SPC = split_c(LA, LB, LA, LB);
     using Q = QClass<LA, LB, LA, LB, SPA, SPC>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -644,7 +649,10 @@ __global__ __launch_bounds__(256) void schwarz_kernel(const ShellPair* pairs, co
 #pragma unroll
                 for (int d = 0; d < Q::ND; d++) {
                     const int ga = SPA ? sa : a, gc = SPC ? sc : c;
-                    if (ga == gc && b == d) m = fmax(m, fabs(eri[((a * Q::NB + b) * Q::NCt + c) * Q::ND + d]));
+                    if (ga == gc && b == d) {
+                        const double s2 = scale[pr.fa + ga] * scale[pr.fb + b];
+                        m = fmax(m, s2 * s2 * fabs(eri[((a * Q::NB + b) * Q::NCt + c) * Q::ND + d]));
+                    }
                 }
     if (m > 0.0) atomicMax(&q2bits[pid], (unsigned long long)__double_as_longlong(m));
 }
@@ -949,33 +957,33 @@ hipError_t hfx_launch_block(int cb, int ck, const BlockArgs& a, hipStream_t s) {
     return BLK_TABLE[cb][ck](a, s);
 }
 hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,
-                              unsigned long long* q2bits, hipStream_t s) {
+                              const double* scale, unsigned long long* q2bits, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
     const unsigned nb = (unsigned)((npairs + 255) / 256);
     switch (cls) {
         case 0:
             hipLaunchKernelGGL((schwarz_kernel<0, 0>), dim3(nb, hfx_nsplit(0, 0)), dim3(256), 0, s, pairs, prims,
-                               list, npairs, q2bits);
+                               list, npairs, scale, q2bits);
             break;
         case 1:
             hipLaunchKernelGGL((schwarz_kernel<1, 0>), dim3(nb, hfx_nsplit(1, 1)), dim3(256), 0, s, pairs, prims,
-                               list, npairs, q2bits);
+                               list, npairs, scale, q2bits);
             break;
         case 2:
             hipLaunchKernelGGL((schwarz_kernel<1, 1>), dim3(nb, hfx_nsplit(2, 2)), dim3(256), 0, s, pairs, prims,
-                               list, npairs, q2bits);
+                               list, npairs, scale, q2bits);
             break;
         case 3:
             hipLaunchKernelGGL((schwarz_kernel<2, 0>), dim3(nb, hfx_nsplit(3, 3)), dim3(256), 0, s, pairs, prims,
-                               list, npairs, q2bits);
+                               list, npairs, scale, q2bits);
             break;
         case 4:
             hipLaunchKernelGGL((schwarz_kernel<2, 1>), dim3(nb, hfx_nsplit(4, 4)), dim3(256), 0, s, pairs, prims,
-                               list, npairs, q2bits);
+                               list, npairs, scale, q2bits);
             break;
         case 5:
             hipLaunchKernelGGL((schwarz_kernel<2, 2>), dim3(nb, hfx_nsplit(5, 5)), dim3(256), 0, s, pairs, prims,
-                               list, npairs, q2bits);
+                               list, npairs, scale, q2bits);
             break;
         default:
             return hipErrorInvalidValue;

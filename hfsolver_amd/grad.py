@@ -1,6 +1,6 @@
 """Analytic RHF nuclear gradient and its torch.autograd.Function.
 
-dE/dX = sum_ij P_ij dH_ij/dX - sum_ij W_ij dS_ij/dX                 (hfx_grad_1e, host)
+dE/dX = sum_ij P_ij dH_ij/dX - sum_ij W_ij dS_ij/dX                 (hfx_grad_1e, GPU)
       + sum_ijkl [1/2 P_ij P_kl - 1/4 P_ik P_jl] d(ij|kl)/dX          (hfx_grad_2e, GPU)
       + dE_nuc/dX,
 with P = 2 C_occ C_occ^T and the energy-weighted density
@@ -45,13 +45,21 @@ class ForceField:
         self.P = None
         self.last = None
 
-    def __call__(self, xyz):
+    def _scf(self, xyz):
         xyz = np.asarray(xyz, dtype=float).reshape(-1, 3)
         if not np.array_equal(xyz, self.engine.basis.xyz):
             self.engine.set_geometry(xyz)
         r = rhf_gauss(self.atno, xyz, engine=self.engine, P0=self.P, **self.kw)
         self.P = r.P
         self.last = r
+        return r
+
+    def energy(self, xyz):
+        """Total energy only (warm-started SCF)."""
+        return self._scf(xyz).Etot
+
+    def __call__(self, xyz):
+        r = self._scf(xyz)
         return r.Etot, rhf_gradient(r)
 
     def close(self):

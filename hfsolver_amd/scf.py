@@ -77,11 +77,11 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
 
     incremental > 0: G(P) = G(P_prev) + G(P - P_prev), the difference built with
     density-weighted screening at that threshold (quartets with Q_ab Q_cd
-    max|dP| below it are skipped); a full build every `rebuild` iterations when
-    rebuild > 0, and full builds once the density change is below 10 tolP, so
-    that the last iterations carry no accumulated screening noise. At
-    convergence E0, F and G are recomputed from one unscreened build of the
-    final input density.
+    max|dP| below it are skipped); a full, unscreened build every `rebuild`
+    iterations when rebuild > 0, and full builds once the density change is
+    below 10 tolP, so that the last iterations (and the C, lam, P they return)
+    carry no screening noise. At convergence E0, F and G are recomputed from one
+    unscreened build of the final input density.
 
     purify: the density of each iteration by canonical purification of the
     orthogonalised Fock matrix (matrix products) instead of its eigenvectors;
@@ -110,8 +110,14 @@ def _doscf_device(H, S, nocc, engine, Nscf, alpha, tolE, tolP, diis, diis_size, 
                 engine.fock_2e_device((P - Pb).contiguous(), dG)
                 G = Gb + dG
             else:
+                # full builds carry no density screening: the last iterations and
+                # the returned C, lam, P come from unscreened Fock matrices
+                if incremental > 0:
+                    engine.set_density_screening(0.0)
                 G = torch.empty_like(G)
                 engine.fock_2e_device(P, G)
+                if incremental > 0:
+                    engine.set_density_screening(incremental)
             Pb, Gb = P, G
             res = _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, callback, alpha, tolE, tolP,
                                dstate)

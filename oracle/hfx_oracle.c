@@ -16,6 +16,9 @@
  *   orc_getInts         qc.c:95-132 + cints.c:327-347 (packed 8-fold int2; 64-bit index)
  *   orc_jk              scf.f90:26-37 / :39-58        (formG: J, K of a density)
  *   orc_eri_bf          one contracted (ij|kl) for arbitrary basis functions
+ *   orc_schwarz,        Schwarz bounds of shell pairs and J, K over the
+ *   orc_jk_screened     Schwarz-screened unique shell quartets (the direct
+ *                       engine's screening rule; same THO integrals as orc_jk)
  *   orc_grad_*          derivative integrals by the centre-derivative identity
  *                       d/dA_x x_A^l e^{-a x_A^2} = 2a x_A^{l+1}(...) - l x_A^{l-1}(...)
  *                       applied to the same THO primitives (no reference
@@ -412,6 +415,110 @@ void orc_jk(int nbf, const int *nprim, const int *istart, const double *x, const
         free(Jt);
         free(Kt);
     }
+}
+
+/* Scatters one unique integral v = (ij|kl) to J and K through its distinct
+ * ordered images: (ab|cd) adds P_cd v to J_ab and P_cb v to K_ad. */
+static void scatter_jk(int nbf, int i, int j, int k, int ll, double v, const double *P, double *J, double *K) {
+    int q[8][4] = {{i, j, k, ll}, {j, i, k, ll}, {i, j, ll, k}, {j, i, ll, k},
+                   {k, ll, i, j}, {ll, k, i, j}, {k, ll, j, i}, {ll, k, j, i}};
+    int seen[8][4], nd = 0;
+    for (int t = 0; t < 8; t++) {
+        int dup = 0;
+        for (int s = 0; s < nd; s++)
+            if (seen[s][0] == q[t][0] && seen[s][1] == q[t][1] && seen[s][2] == q[t][2] && seen[s][3] == q[t][3])
+                dup = 1;
+        if (dup) continue;
+        memcpy(seen[nd++], q[t], sizeof(q[t]));
+        int a = q[t][0], b = q[t][1], c = q[t][2], d = q[t][3];
+        J[a + (size_t)b * nbf] += P[c + (size_t)d * nbf] * v;
+        K[a + (size_t)d * nbf] += P[c + (size_t)b * nbf] * v;
+    }
+}
+
+/* Schwarz bounds of every shell pair (A >= B): Q_AB = sqrt(max |(ab|ab)|) over
+ * the functions a of A and b of B, from the THO integrals. Shell s spans the
+ * functions [first[s], first[s] + nf[s]). Q is packed: Q[A(A+1)/2 + B]. */
+void orc_schwarz(int nbf, const int *nprim, const int *istart, const double *x, const double *y, const double *z,
+                 const int *l, const int *m, const int *n, const double *coef, const double *alpha, int nsh,
+                 const int *first, const int *nf, double *Q) {
+    orc_basis B = mkbasis(nbf, nprim, istart, x, y, z, l, m, n, coef, alpha);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int A = 0; A < nsh; A++)
+        for (int Bs = 0; Bs <= A; Bs++) {
+            double mx = 0.0;
+            for (int a = first[A]; a < first[A] + nf[A]; a++)
+                for (int b = first[Bs]; b < first[Bs] + nf[Bs]; b++) {
+                    double v = fabs(eri_bf(&B, a, b, a, b));
+                    if (v > mx) mx = v;
+                }
+            Q[(size_t)A * (A + 1) / 2 + Bs] = sqrt(mx);
+        }
+}
+
+/* J and K of a density over Schwarz-screened shell quartets (the direct
+ * engine's screening rule, no reference counterpart: the reference's formG,
+ * scf.f90:26-37, contracts every stored integral). Unique shell quartets
+ * (AB|CD) with A >= B, C >= D and pair index AB >= CD; a quartet is skipped
+ * when Q_AB * Q_CD < thr (Q from orc_schwarz). Inside a quartet the unique
+ * function quartets are those with i >= j when A == B, k >= l when C == D, and
+ * function-pair index ij >= kl when AB == CD; each is evaluated once by THO
+ * (same integrals as orc_jk) and scattered to its distinct images.
+ * Returns the number of shell quartets evaluated. P, J, K column-major. */
+double orc_jk_screened(int nbf, const int *nprim, const int *istart, const double *x, const double *y,
+                       const double *z, const int *l, const int *m, const int *n, const double *coef,
+                       const double *alpha, int nsh, const int *first, const int *nf, const double *Q, double thr,
+                       const double *P, double *J, double *K, int nthreads) {
+    orc_basis B = mkbasis(nbf, nprim, istart, x, y, z, l, m, n, coef, alpha);
+    size_t nn = (size_t)nbf * nbf;
+    long npair = (long)nsh * (nsh + 1) / 2;
+    memset(J, 0, nn * sizeof(double));
+    memset(K, 0, nn * sizeof(double));
+    double count = 0.0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel reduction(+ : count)
+    {
+        double *Jt = (double *)calloc(nn, sizeof(double));
+        double *Kt = (double *)calloc(nn, sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (long ab = npair - 1; ab >= 0; ab--) {
+            int A = (int)((sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
+            while ((long)A * (A + 1) / 2 > ab) A--;
+            while ((long)(A + 1) * (A + 2) / 2 <= ab) A++;
+            int Bs = (int)(ab - (long)A * (A + 1) / 2);
+            for (long cd = 0; cd <= ab; cd++) {
+                if (Q[ab] * Q[cd] < thr) continue;
+                int C = (int)((sqrt(8.0 * cd + 1.0) - 1.0) / 2.0);
+                while ((long)C * (C + 1) / 2 > cd) C--;
+                while ((long)(C + 1) * (C + 2) / 2 <= cd) C++;
+                int D = (int)(cd - (long)C * (C + 1) / 2);
+                count += 1.0;
+                for (int i = first[A]; i < first[A] + nf[A]; i++)
+                    for (int j = first[Bs]; j < first[Bs] + nf[Bs]; j++) {
+                        if (A == Bs && j > i) continue;
+                        long fij = (long)i * (i + 1) / 2 + j;
+                        for (int k = first[C]; k < first[C] + nf[C]; k++)
+                            for (int ll = first[D]; ll < first[D] + nf[D]; ll++) {
+                                if (C == D && ll > k) continue;
+                                if (ab == cd && (long)k * (k + 1) / 2 + ll > fij) continue;
+                                scatter_jk(nbf, i, j, k, ll, eri_bf(&B, i, j, k, ll), P, Jt, Kt);
+                            }
+                    }
+            }
+        }
+#pragma omp critical
+        {
+            for (size_t t = 0; t < nn; t++) {
+                J[t] += Jt[t];
+                K[t] += Kt[t];
+            }
+        }
+        free(Jt);
+        free(Kt);
+    }
+    return count;
 }
 
 /* ---------------------------------------------------- gradients --------- */

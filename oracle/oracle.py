@@ -18,6 +18,9 @@ hfsolver's src/):
   libint_shell_walk gaussians_libint.f90:70-124 (shells from per-function powers)
   transform_int2    mbpt.f90:11-79      (AO -> MO packed integrals)
   mbpt2             mbpt.f90:154-159 + mbpt2.f90:8-31 (closed-shell second order)
+  jk_screened       scf.f90:26-37 semantics over Schwarz-screened unique shell
+                    quartets (no reference counterpart for the screening)
+  run_reference     runs the compiled reference itself (oracle/_ref/ref_driver)
 The C half (hfx_oracle.c, loaded through ctypes) holds the Boys function and
 the THO integrals.
 
@@ -31,6 +34,9 @@ integrals, and the identity transform.
 import ctypes
 import math
 import os
+import shutil
+import subprocess
+import tempfile
 
 import numpy as np
 import scipy.linalg
@@ -72,6 +78,10 @@ def lib():
         L.orc_deri_bf.restype = None
         L.orc_eri_quartets.argtypes = [i] + [p] * 10 + [i, p, p, i]
         L.orc_eri_quartets.restype = d
+        L.orc_schwarz.argtypes = [i] + [p] * 10 + [i, p, p, p]
+        L.orc_schwarz.restype = None
+        L.orc_jk_screened.argtypes = [i] + [p] * 10 + [i, p, p, p, d, p, p, p, i]
+        L.orc_jk_screened.restype = d
         _lib = L
     return _lib
 
@@ -343,6 +353,39 @@ def jk(B, P, nthreads=0):
     return J, K
 
 
+def shells(B):
+    """(first function, function count) of every shell, in function order
+    (the libint plug-in's walk over the per-function powers)."""
+    walk = libint_shell_walk(np.stack([B.l, B.m, B.n]))
+    first = np.array([f for f, _ in walk], dtype=np.int32)
+    nf = np.array([(L + 1) * (L + 2) // 2 for _, L in walk], dtype=np.int32)
+    return first, nf
+
+
+def schwarz(B, nthreads=0):
+    """Packed Schwarz bounds Q[A(A+1)/2 + B] = sqrt(max |(ab|ab)|) of all shell pairs A >= B."""
+    first, nf = shells(B)
+    ns = len(first)
+    Q = np.zeros(ns * (ns + 1) // 2)
+    lib().orc_schwarz(*B.cargs(), ns, _ptr(first), _ptr(nf), _ptr(Q))
+    return Q
+
+
+def jk_screened(B, P, thr=1e-12, nthreads=0, Q=None):
+    """J, K over the unique shell quartets with Q_AB Q_CD >= thr (the direct
+    engine's Schwarz screening), THO integrals; returns (J, K, quartets)."""
+    n = B.nbf
+    first, nf = shells(B)
+    if Q is None:
+        Q = schwarz(B, nthreads)
+    Pf = np.asfortranarray(P, dtype=float)
+    J = np.zeros((n, n), order="F")
+    K = np.zeros((n, n), order="F")
+    nq = lib().orc_jk_screened(*B.cargs(), len(first), _ptr(first), _ptr(nf), _ptr(np.ascontiguousarray(Q)),
+                               float(thr), _ptr(Pf), _ptr(J), _ptr(K), int(nthreads))
+    return J, K, nq
+
+
 def formG(B, P, nthreads=0):
     J, K = jk(B, P, nthreads)
     return J - 0.5 * K
@@ -438,3 +481,68 @@ def gradient(r, nthreads=0):
                 d = B.xyz[i] - B.xyz[j]
                 gn[i] -= B.atno[i] * B.atno[j] * d / np.linalg.norm(d) ** 3
     return (g1 + g2).reshape(nat, 3) + gn
+
+
+# ------------------------------------------- the compiled reference itself ---
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+
+def _ffloat(tok):
+    """Fortran ES output drops the 'E' for 3-digit exponents (8.99-115)."""
+    try:
+        return float(tok)
+    except ValueError:
+        k = max(tok.rfind("-"), tok.rfind("+"))
+        return float(tok[:k] + "E" + tok[k:])
+
+
+def parse_ref_output(path):
+    """Blocks "key count" + count values of ref_driver's output file; the int2
+    sample (count -1) becomes int2_idx (1-based) and int2_val."""
+    d = {}
+    lines = open(path).read().split("\n")
+    i = 0
+    while i < len(lines):
+        t = lines[i].split()
+        if not t:
+            i += 1
+            continue
+        key, cnt = t[0], int(t[1])
+        i += 1
+        if cnt == -1:
+            rows = [r.split() for r in lines[i:] if r.strip()]
+            d["int2_idx"] = np.array([[int(r[0]), int(r[1]), int(r[2]), int(r[3])] for r in rows], dtype=np.int32)
+            d["int2_val"] = np.array([_ffloat(r[4]) for r in rows])
+            break
+        d[key] = np.array([_ffloat(x) for x in lines[i:i + cnt]])
+        i += cnt
+    return d
+
+
+def run_reference(atno, xyz, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, ndump=0, timeout=None):
+    """Runs the compiled reference (oracle/_ref/ref_driver: gaussints, doscf,
+    energies, drivers.f90:29-41) on one core in a temporary directory holding
+    the 6-31G** file it reads (basis.f90:210). Returns (parsed output, the
+    per-iteration E0 printed by doscf, scf.f90:114)."""
+    if not os.path.exists(REF_DRIVER):
+        raise RuntimeError("oracle/_ref/ref_driver is not built (make -f oracle/Makefile.ref)")
+    with tempfile.TemporaryDirectory(prefix="hfx_ref_") as work:
+        shutil.copy(BASIS_FILE, os.path.join(work, "p631ss.txt"))
+        inp = os.path.join(work, "case.in")
+        out = os.path.join(work, "case.out")
+        with open(inp, "w") as f:
+            f.write("%d\n" % len(atno))
+            for z, r in zip(atno, np.asarray(xyz, dtype=float).reshape(-1, 3)):
+                f.write("%d %.17e %.17e %.17e\n" % (z, r[0], r[1], r[2]))
+            f.write("%d %.17e %.17e %.17e\n%d\n" % (Nscf, alpha, tolE, tolP, ndump))
+        env = dict(os.environ, MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
+        # doscf keeps its n^4 TT array on the stack (scf.f90:79)
+        log = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec %s %s %s" % (REF_DRIVER, inp, out)],
+                             cwd=work, env=env, capture_output=True, text=True, check=True, timeout=timeout).stdout
+        d = parse_ref_output(out)
+    e_iter = []
+    for line in log.split("\n"):
+        t = line.split()
+        if len(t) == 2 and t[0].isdigit():
+            e_iter.append(float(t[1]))
+    return d, e_iter

@@ -7,6 +7,9 @@
 ! G(P) formed from the reference's packed int2 with the formG2 loop semantics
 ! (src/scf.f90:39-58), and a list of packed ERIs.
 !
+! It also records the wall time of gaussints and of doscf (system_clock), so
+! that the reference's CPU path can be timed beside the GPU engine.
+!
 ! Usage: ref_driver <input> <output>
 !   input:  nat / (Z x y z) * nat [bohr] / Nscf alpha tolE tolP / nint2_dump
 !   output: plain text, one "key n" header followed by n values per block.
@@ -21,7 +24,8 @@ integer :: nat, i, j, k, l, n, Nscf, u, nocc, ndump, m
 integer, allocatable :: atno(:)
 real(dp), allocatable :: xyz(:, :), C(:, :), lam(:), int2(:), S(:, :), &
     T(:, :), V(:, :), P(:, :), H(:, :), G(:, :), Jm(:, :), Km(:, :)
-real(dp) :: alpha, tolE, tolP, E0, Enuc, Ekin, Etot
+real(dp) :: alpha, tolE, tolP, E0, Enuc, Ekin, Etot, t_ints, t_scf
+integer(8) :: c0, c1, c2, crate
 character(len=1024) :: inpath, outpath
 
 call get_command_argument(1, inpath)
@@ -36,12 +40,17 @@ read(u, *) Nscf, alpha, tolE, tolP
 read(u, *) ndump
 close(u)
 
+call system_clock(c0, crate)
 call gaussints(atno, xyz, S, T, V, int2)
+call system_clock(c1)
 n = size(S, 1)
 allocate(H(n, n), P(n, n), C(n, n), lam(n), G(n, n), Jm(n, n), Km(n, n))
 H = T + V
 nocc = sum(atno) / 2
 call doscf(H, int2, S, nocc, Nscf, tolE, tolP, alpha, C, P, lam, E0)
+call system_clock(c2)
+t_ints = real(c1 - c0, dp) / real(crate, dp)
+t_scf = real(c2 - c1, dp) / real(crate, dp)
 Enuc = get_nuclear_energy(atno, xyz)
 Ekin = kinetic_energy(P, T)
 Etot = E0 + Enuc
@@ -68,6 +77,10 @@ call wscal("E0", E0)
 call wscal("Enuc", Enuc)
 call wscal("Ekin", Ekin)
 call wscal("Etot", Etot)
+! wall times: gaussints (S, T, V and the packed int2) and doscf (TT precompute
+! plus every SCF iteration)
+call wscal("t_gaussints", t_ints)
+call wscal("t_doscf", t_scf)
 call wvec("lam", lam)
 call wvec("S", reshape(S, [n*n]))
 call wvec("T", reshape(T, [n*n]))

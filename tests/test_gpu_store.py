@@ -28,7 +28,7 @@ def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-@pytest.mark.parametrize("name", ["h2o_config0", "ch4", "water2_seed2"])
+@pytest.mark.parametrize("name", ["h2o_config0", "ch4", "water2_seed2", "water4_seed4"])
 def test_int2_matches_reference_entries(golden, name):
     g = golden(name)
     say("int2 vs reference entries,", name)

@@ -6,7 +6,11 @@
 - Energies: the reference's literature known answers
   (src/tests/test_gaussians_molecules.f90:106,110,183).
 - Packed 8-fold index: counts and permutation symmetry (src/tests/test_ijkl.f90).
-- Gradient: central finite differences of the oracle's own energy.
+- Gradient: central finite differences of the oracle's own energy and of the
+  compiled reference's energy (tests/golden/water2_fd.npz).
+- Screened J/K (the direct engine's Schwarz rule): its unscreened limit and
+  the four-water case against the compiled reference, quartet counts, and the
+  Cauchy-Schwarz bound of sampled integrals.
 """
 import math
 
@@ -67,7 +71,7 @@ def test_intindex_symmetry_and_count():
     assert max(seen) == len(seen) - 1
 
 
-@pytest.mark.parametrize("name", ["h2o_testgeom", "h2o_config0", "ch4", "water2_seed2"])
+@pytest.mark.parametrize("name", ["h2o_testgeom", "h2o_config0", "ch4", "water2_seed2", "water4_seed4"])
 def test_one_electron_matches_reference(golden, name):
     g = golden(name)
     B = O.Basis(g["atno"], g["xyz"])
@@ -181,3 +185,55 @@ def test_libint_shell_walk_counts():
     assert len(walk) == 12 and lams.count(0) == 7 and lams.count(1) == 4 and lams.count(2) == 1
     nxt = [f + (lam + 1) * (lam + 2) // 2 for f, lam in walk]
     assert [f for f, _ in walk][1:] == nxt[:-1] and nxt[-1] == power.shape[1] == 25
+
+
+def test_screened_jk_unscreened_limit_matches_reference(golden):
+    """orc_jk_screened at threshold 0 visits every unique shell quartet once:
+    it reproduces the reference's J, K (formG over its stored int2) and the
+    shell-quartet count m(m+1)/2 of the m = ns(ns+1)/2 shell pairs."""
+    g = golden("water2_seed2")
+    B = O.Basis(g["atno"], g["xyz"])
+    J, K, nq = O.jk_screened(B, g["P"], thr=0.0)
+    ns = len(O.shells(B)[0])
+    m = ns * (ns + 1) // 2
+    assert nq == m * (m + 1) // 2
+    assert np.abs(J - g["J"]).max() < 1e-12
+    assert np.abs(K - g["K"]).max() < 1e-12
+
+
+def test_screened_jk_matches_reference_at_four_waters(golden):
+    """Schwarz screening at 1e-12 on four waters (n = 100, the largest cluster
+    the reference runs): the reference's J, K within the screening error, the
+    quartet count equal to a direct count over the bounds, and every sampled
+    integral within its Cauchy-Schwarz bound."""
+    g = golden("water4_seed4")
+    B = O.Basis(g["atno"], g["xyz"])
+    Q = O.schwarz(B)
+    J, K, nq = O.jk_screened(B, g["P"], thr=1e-12, Q=Q)
+    cnt = sum(int(np.count_nonzero(Q[:i + 1] * q >= 1e-12)) for i, q in enumerate(Q))
+    assert nq == cnt and cnt < len(Q) * (len(Q) + 1) // 2
+    assert np.abs(J - g["J"]).max() < 1e-10
+    assert np.abs(K - g["K"]).max() < 1e-10
+    first, nf = O.shells(B)
+    rng = np.random.default_rng(4)
+    ns = len(first)
+    for A, Bs, C, D in rng.integers(0, ns, size=(30, 4)):
+        qa = Q[max(A, Bs) * (max(A, Bs) + 1) // 2 + min(A, Bs)]
+        qc = Q[max(C, D) * (max(C, D) + 1) // 2 + min(C, D)]
+        v = O.eri(B, first[A], first[Bs] + nf[Bs] - 1, first[C], first[D])
+        assert abs(v) <= qa * qc * (1 + 1e-12)
+
+
+def test_gradient_matches_reference_energy_differences(golden):
+    """The oracle's analytic gradient of the water dimer against the compiled
+    reference's own central differences of the total energy (h = 1e-4 bohr,
+    12 seeded components, tests/golden/water2_fd.npz)."""
+    f = golden("water2_fd")
+    g0 = golden("water2_seed2")
+    assert np.array_equal(f["xyz"], g0["xyz"])
+    r = O.rhf_gauss(f["atno"], f["xyz"], Nscf=200, alpha=0.5, tolE=1e-12, tolP=1e-10, P0=g0["P"])
+    grad = O.gradient(r)
+    h = float(f["h"])
+    for c, ep, em in zip(f["comps"], f["E_plus"], f["E_minus"]):
+        a, d = divmod(int(c), 3)
+        assert abs((ep - em) / (2 * h) - grad[a, d]) < 1e-7
