@@ -61,6 +61,7 @@ struct hfx_engine {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
+    void* scratch = nullptr;   // device address of this GPU's allocation-free scratch array
     // basis
     int nbf = 0;
     int nat = 0;
@@ -922,6 +923,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (h->opts.device < 0 || h->opts.device >= ndev) return fail(h, "hfx_create: device %d out of range", h->opts.device);
     h->device = h->opts.device;
     HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hfx_scratch_address(h->device, &h->scratch));
     HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
@@ -1353,6 +1355,145 @@ int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
                                           h->scale[h->shells[C].f0 + c] * h->scale[h->shells[D].f0 + d];
                         dst[((a * nB + b) * nC + c) * nD + d] = sc * src[((x1 * n2 + x2) * n3 + x3) * n4 + x4];
                     }
+    }
+    return 0;
+}
+
+int hfx_eri_shells_noalloc(hfx_t* h, int nq, const int* shells, double* out) {
+    if (!h || nq < 0 || (nq > 0 && (!shells || !out)))
+        return h ? fail(h, "hfx_eri_shells_noalloc: invalid arguments") : 1;
+    if (nq == 0) return 0;
+    HIPCHK(h, hipSetDevice(h->device));
+    if (!h->scratch) return fail(h, "hfx_eri_shells_noalloc: no scratch address for device %d", h->device);
+    const int ns = (int)h->shells.size();
+    long long tot = 0;
+    for (int q = 0; q < nq; q++) {
+        const int* r = shells + 4 * q;
+        for (int k = 0; k < 4; k++)
+            if (r[k] < 0 || r[k] >= ns) return fail(h, "hfx_eri_shells_noalloc: shell index out of range in quartet %d", q);
+        tot += (long long)ncart(h->shells[r[0]].L) * ncart(h->shells[r[1]].L) * ncart(h->shells[r[2]].L) *
+               ncart(h->shells[r[3]].L);
+    }
+    // scratch layout: bra, ket, grouped quartet ids (int), block offsets (int64), integrals
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t o_ket = up((size_t)nq * sizeof(int)), o_q = o_ket + up((size_t)nq * sizeof(int));
+    const size_t o_off = o_q + up((size_t)nq * sizeof(int)), o_out = o_off + up((size_t)nq * sizeof(long long));
+    const size_t need = o_out + (size_t)tot * sizeof(double);
+    if (need > HFX_SCRATCH_BYTES)
+        return fail(h, "hfx_eri_shells_noalloc: %d quartets with %lld integrals need %zu bytes of scratch, more than "
+                       "its %zu bytes (use hfx_eri_shells)", nq, tot, need, HFX_SCRATCH_BYTES);
+    unsigned char* base = (unsigned char*)h->scratch;
+    int* d_bra = (int*)base;
+    int* d_ket = (int*)(base + o_ket);
+    int* d_q = (int*)(base + o_q);
+    long long* d_off = (long long*)(base + o_off);
+    double* d_out = (double*)(base + o_out);
+    // oriented shell pairs of quartet q (as hfx_eri_shells): bit 0 of perm = bra
+    // swapped within its pair, bit 1 = ket swapped, bit 2 = bra <-> ket
+    auto plan = [&](int q, int& p1, int& p2, int& pm) {
+        const int A = shells[4 * q], B = shells[4 * q + 1], C = shells[4 * q + 2], D = shells[4 * q + 3];
+        p1 = A >= B ? A * (A + 1) / 2 + B : B * (B + 1) / 2 + A;
+        p2 = C >= D ? C * (C + 1) / 2 + D : D * (D + 1) / 2 + C;
+        pm = (h->pairs[p1].A != A ? 1 : 0) | (h->pairs[p2].A != C ? 2 : 0);
+        if (h->pairs[p1].cls < h->pairs[p2].cls) {
+            std::swap(p1, p2);
+            pm |= 4;
+        }
+    };
+    // plans staged through fixed host buffers: the call allocates no memory
+    constexpr int CH = 512;
+    int hb[CH], hk[CH];
+    long long ho[CH];
+    long long run = 0;
+    for (int q0 = 0; q0 < nq; q0 += CH) {
+        const int m = std::min(CH, nq - q0);
+        for (int t = 0; t < m; t++) {
+            const int q = q0 + t;
+            int pm;
+            plan(q, hb[t], hk[t], pm);
+            ho[t] = run;
+            const int* r = shells + 4 * q;
+            run += (long long)ncart(h->shells[r[0]].L) * ncart(h->shells[r[1]].L) * ncart(h->shells[r[2]].L) *
+                   ncart(h->shells[r[3]].L);
+        }
+        // copies on the engine's stream (ordered with the launches); the staging
+        // buffers are reused only after they have been read
+        HIPCHK(h, hipMemcpyAsync(d_bra + q0, hb, m * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(d_ket + q0, hk, m * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(d_off + q0, ho, m * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    // one launch per class combination over its quartets (ids grouped in d_q)
+    int pos = 0;
+    for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
+        for (int ck = 0; ck <= cb; ck++) {
+            const int start = pos;
+            int fill = 0;
+            for (int q = 0; q < nq; q++) {
+                int p1, p2, pm;
+                plan(q, p1, p2, pm);
+                if (h->pairs[p1].cls != cb || h->pairs[p2].cls != ck) continue;
+                hb[fill++] = q;
+                if (fill == CH) {
+                    HIPCHK(h, hipMemcpyAsync(d_q + pos, hb, fill * sizeof(int), hipMemcpyHostToDevice, h->stream));
+                    HIPCHK(h, hipStreamSynchronize(h->stream));
+                    pos += fill;
+                    fill = 0;
+                }
+            }
+            if (fill > 0) {
+                HIPCHK(h, hipMemcpyAsync(d_q + pos, hb, fill * sizeof(int), hipMemcpyHostToDevice, h->stream));
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                pos += fill;
+            }
+            if (pos == start) continue;
+            BlockArgs ba;
+            ba.pairs = h->d_pairs;
+            ba.prims = h->d_prims;
+            ba.bra = d_bra;
+            ba.ket = d_ket;
+            ba.off = d_off;
+            ba.qidx = d_q + start;
+            ba.nq = pos - start;
+            ba.out = d_out;
+            ba.packed = 0;
+            ba.scale = nullptr;
+            HIPCHK(h, hfx_launch_block(cb, ck, ba, h->stream));
+        }
+    if (pos != nq) return fail(h, "hfx_eri_shells_noalloc: %d of %d quartets planned", pos, nq);
+    HIPCHK(h, hipMemcpyAsync(out, d_out, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    // undo the orientation in place (block by block) and apply the function scales
+    double blk[HFX_MAX_BLOCK];
+    long long o = 0;
+    for (int q = 0; q < nq; q++) {
+        const int A = shells[4 * q], B = shells[4 * q + 1], C = shells[4 * q + 2], D = shells[4 * q + 3];
+        const int nA = ncart(h->shells[A].L), nB = ncart(h->shells[B].L), nC = ncart(h->shells[C].L),
+                  nD = ncart(h->shells[D].L);
+        const int sz = nA * nB * nC * nD;
+        int p1, p2, pm;
+        plan(q, p1, p2, pm);
+        const ShellPair& pb = h->pairs[p1];
+        const ShellPair& pk = h->pairs[p2];
+        const int n2 = ncart(h->shells[pb.B].L), n3 = ncart(h->shells[pk.A].L), n4 = ncart(h->shells[pk.B].L);
+        double* dst = out + o;
+        std::memcpy(blk, dst, sz * sizeof(double));
+        for (int a = 0; a < nA; a++)
+            for (int b = 0; b < nB; b++)
+                for (int c = 0; c < nC; c++)
+                    for (int d = 0; d < nD; d++) {
+                        int x1 = a, x2 = b, x3 = c, x4 = d;
+                        if (pm & 1) std::swap(x1, x2);
+                        if (pm & 2) std::swap(x3, x4);
+                        if (pm & 4) {
+                            std::swap(x1, x3);
+                            std::swap(x2, x4);
+                        }
+                        const double sc = h->scale[h->shells[A].f0 + a] * h->scale[h->shells[B].f0 + b] *
+                                          h->scale[h->shells[C].f0 + c] * h->scale[h->shells[D].f0 + d];
+                        dst[((a * nB + b) * nC + c) * nD + d] = sc * blk[((x1 * n2 + x2) * n3 + x3) * n4 + x4];
+                    }
+        o += sz;
     }
     return 0;
 }

@@ -242,6 +242,15 @@ inline size_t hfx_grad_row_lds(int nrow, int ncols, int R, int dlds, bool fp32, 
 hipError_t hfx_launch_grad_rows(int cls_bra, int cls_ket, bool fp32, const GradRowArgs& a, int nblocks,
                                 hipStream_t s);
 hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
+// Device scratch of the allocation-free entry point: a file-scope array of
+// HFX_SCRATCH_BYTES per GPU. hfx_scratch_address returns its device address
+// for `device`, which must be the current device (looked up once per device).
+static constexpr size_t HFX_SCRATCH_BYTES = (size_t)16 << 20;
+static constexpr int HFX_MAX_DEVICES = 64;
+// largest shell-quartet block, (dd|dd): 6^4 integrals
+static constexpr int HFX_MAX_BLOCK = ((HFX_LMAX + 1) * (HFX_LMAX + 2) / 2) * ((HFX_LMAX + 1) * (HFX_LMAX + 2) / 2) *
+                                     ((HFX_LMAX + 1) * (HFX_LMAX + 2) / 2) * ((HFX_LMAX + 1) * (HFX_LMAX + 2) / 2);
+hipError_t hfx_scratch_address(int device, void** ptr);
 // Q2[pair] = max over the diagonal integrals (ab|ab) of the pair (squared bound)
 hipError_t hfx_launch_schwarz(int cls, const ShellPair* pairs, const PrimPair* prims, const int* list, int npairs,
                               const double* scale, unsigned long long* q2bits, hipStream_t s);

@@ -11,6 +11,8 @@
 //                G = J - K/2.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "grad_rys.h"
 #include "hfx_internal.h"
 #include "one_e.h"
@@ -1088,6 +1090,34 @@ hipError_t hfx_launch_finish(const double* Jb, const double* Kb, const double* s
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(finish_kernel, dim3(nb), dim3(256), 0, s, Jb, Kb, scale, G, J, K, n);
     return hipGetLastError();
+}
+
+// Scratch of the allocation-free entry point (hfx_eri_shells_noalloc): the
+// quartet plans and integral blocks of one call, in a file-scope device array
+// of HFX_SCRATCH_BYTES (one copy per GPU, held until the process ends). In
+// host code the array's name is a host-side stand-in, never a device address:
+// hfx_scratch_address looks the address up once per device, with that device
+// current, and callers use only the address it returns.
+static __device__ __align__(256) unsigned char scratch[HFX_SCRATCH_BYTES];
+
+hipError_t hfx_scratch_address(int device, void** ptr) {
+    static std::mutex mu;
+    static void* addr[HFX_MAX_DEVICES] = {};
+    if (!ptr || device < 0 || device >= HFX_MAX_DEVICES) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
 }
 
 #endif  // HFX_PART == 6

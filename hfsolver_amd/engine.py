@@ -46,6 +46,7 @@ SIGNATURES = {
     "hfx_grad_1e": (_I, [_P, _P, _P, _P]),
     "hfx_set_geometry": (_I, [_P, _P]),
     "hfx_eri_shells": (_I, [_P, _I, _P, _P]),
+    "hfx_eri_shells_noalloc": (_I, [_P, _I, _P, _P]),
     "hfx_int2": (_I, [_P, _P, ctypes.c_longlong]),
     "hfx_transform_int2": (_I, [_P, _P, _P, _P, ctypes.c_longlong]),
     "hfx_shells": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
@@ -182,14 +183,16 @@ class Engine:
         self._check(lib().hfx_set_geometry(self._h, _ptr(xyz)))
         self.basis.move(xyz)
 
-    def eri_shells(self, quartets):
-        """Integral blocks of explicit shell quartets [(A,B,C,D), ...] -> list of arrays."""
+    def eri_shells(self, quartets, noalloc=False):
+        """Integral blocks of explicit shell quartets [(A,B,C,D), ...] -> list of arrays.
+        noalloc: through hfx_eri_shells_noalloc (fixed device scratch, no allocation)."""
         q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1, 4))
         _, _, Ls, _ = self.shells()
         nc = lambda l: (l + 1) * (l + 2) // 2
         sizes = [nc(Ls[a]) * nc(Ls[b]) * nc(Ls[c]) * nc(Ls[d]) for a, b, c, d in q]
         out = np.zeros(int(sum(sizes)))
-        self._check(lib().hfx_eri_shells(self._h, len(q), _ptr(q), _ptr(out)))
+        fn = lib().hfx_eri_shells_noalloc if noalloc else lib().hfx_eri_shells
+        self._check(fn(self._h, len(q), _ptr(q), _ptr(out)))
         res, o = [], 0
         for (a, b, c, d), s in zip(q, sizes):
             res.append(out[o:o + s].reshape(nc(Ls[a]), nc(Ls[b]), nc(Ls[c]), nc(Ls[d])))

@@ -126,6 +126,22 @@ int hfx_set_geometry(hfx_t *h, const double *xyz);
 int hfx_eri_shells(hfx_t *h, int nq, const int *shells, double *out);
 
 /*
+ * hfx_eri_shells without any memory allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, kernels and results as hfx_eri_shells, but the quartet plans and
+ * the integral blocks are staged in a fixed device scratch array of 16 MiB per
+ * GPU (allocated with the library's code object, held until the process ends)
+ * instead of device memory allocated per call; no hipMalloc, hipFree or host
+ * heap allocation happens. A request whose plans and integrals do not fit
+ * (about 20 + 8 x integrals bytes per quartet, e.g. 1 600 (dd|dd) quartets)
+ * fails with an error naming the sizes. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (from several host threads or
+ * handles) race and must be serialised by the caller.
+ */
+int hfx_eri_shells_noalloc(hfx_t *h, int nq, const int *shells, double *out);
+
+/*
  * Store mode: every unique two-electron integral over the basis functions,
  * written into the 8-fold packed array int2[size], size = m(m+1)/2 with
  * m = nbf(nbf+1)/2, at the 0-based index of src/cints.c:327-347
