@@ -8,8 +8,12 @@ the C ABI's hfx_fock_2e_device. The headline value is the mixed-precision form
 all-FP64 form is timed beside it. Reported beside: |E_tot(mixed SCF) -
 E_tot(FP64 SCF)|, the first-order |dE| = |1/2 tr P (G_mixed - G_fp64)| and
 max |dF_ij| at the density, the rate with host buffers (PCIe included), the
-analytic-gradient times, and the oracle timed on a sample of the same
-quartets (cpu_baseline).
+analytic gradient (time and roofline of the derivative kernels), the
+mixed-precision force workload (10 warm-started energy + force evaluations on
+perturbed geometries, against the FP64 engine), and the CPU baselines: the
+oracle (THO restatement of the reference's integrals) on a sample of the same
+quartets and on one full 16-water build, and the compiled reference's own
+Fortran path on 1 and 2 waters (one core; it cannot run 16 or 64 waters).
 
     python bench.py [--steps K] [--warmup W] [--nwater 64] [--density scf|core]
 """
@@ -24,20 +28,24 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64 = 78.6e12   # MI355X vector / matrix FP64, FLOP/s
-PEAK_FP32 = 157.3e12  # MI355X vector FP32 (packed), FLOP/s
+PEAK_FP64 = 78.6e12   # MI355X vector FP64, FLOP/s
+# The kernels are built without packed FP32 instructions: single-precision FMAs
+# then issue at the FP64 rate, so 78.6 TFLOP/s is the peak for both parts of the
+# mixed build (the packed-FP32 figure, 157.3, does not apply).
+PEAK_VALU = PEAK_FP64
 
 
 TRAFFIC_FILE = "profiles/jk_fock_64w.md"
 
 
-def measured_traffic():
-    """HBM bytes of one FP64 build of the 64-water workload (FETCH_SIZE + WRITE_SIZE
+def measured_traffic(key):
+    """HBM bytes of one build of the 64-water workload (FETCH_SIZE + WRITE_SIZE
     of its J/K dispatches, scaled by a plain-copy calibration), as recorded by the
-    rocprofv3 --pmc runs summarised in TRAFFIC_FILE; None when absent."""
+    rocprofv3 --pmc runs summarised in TRAFFIC_FILE under `key`
+    (traffic_bytes_per_fp64_build / traffic_bytes_per_mixed_build); None when absent."""
     try:
         for line in open(os.path.join(ROOT, TRAFFIC_FILE)):
-            if line.startswith("traffic_bytes_per_fp64_build:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":")[1])
     except OSError:
         pass
@@ -90,6 +98,67 @@ def cpu_baseline(eng, basis, nq_total, target_s=12.0, seed=0):
                       "THO restatement of cints.c (%.1f s), extrapolated to %.4g quartets" % (k, t, nq_total)}
 
 
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def reference_cpu(nwaters):
+    """The compiled reference's own RHF path (oracle/_ref/ref_driver: gaussints,
+    then doscf with its n^4 TT tensor, src/drivers.f90:29-41, src/scf.f90:72-136)
+    on one core for the given numbers of seeded waters. Above n = 120 (more than
+    four waters) the reference stops (src/basis.f90:537): 16 and 64 waters are
+    reported as not runnable."""
+    from hfsolver_amd.clusters import water_cluster
+    from oracle import oracle as O
+    if not os.path.exists(O.REF_DRIVER):
+        return {"status": "oracle/_ref/ref_driver not built"}
+    runs = []
+    for nw in nwaters:
+        atno, xyz = water_cluster(nw, seed=nw)
+        t0 = time.perf_counter()
+        d, e_iter = O.run_reference(atno, xyz, Nscf=300, alpha=0.5, tolE=1e-10, tolP=1e-8)
+        wall = time.perf_counter() - t0
+        tg, ts, it = float(d["t_gaussints"][0]), float(d["t_doscf"][0]), len(e_iter)
+        runs.append({"waters": nw, "nbf": int(d["n"][0]), "gaussints_s": tg, "doscf_s": ts, "scf_iterations": it,
+                     "rhf_gauss_wall_s": wall, "fock_builds_per_s_stored_integrals": it / ts,
+                     "first_fock_build_s_with_integrals": tg + ts / it})
+        log("  reference Fortran, %d waters (n = %d): gaussints %.2f s, doscf %.2f s for %d iterations"
+            % (nw, runs[-1]["nbf"], tg, ts, it))
+    return {"runs": runs, "cores": 1, "threads": "MKL_NUM_THREADS=1 (the path has no OpenMP)",
+            "16_waters": "n/a (cannot run: n = 400 > 120, src/basis.f90:537)",
+            "64_waters": "n/a (cannot run: n = 1600 > 120, src/basis.f90:537)"}
+
+
+def restatement_full_build(nwater=16, seed=16):
+    """One complete, unextrapolated FP64 J/K build of nwater waters by the
+    oracle (THO restatement of cints.c over the Schwarz-screened unique shell
+    quartets, threshold 1e-12) on all host cores."""
+    from hfsolver_amd.clusters import water_cluster
+    from oracle import oracle as O
+    atno, xyz = water_cluster(nwater, seed=seed)
+    B = O.Basis(atno, xyz)
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((B.nbf, 8))
+    P = X @ X.T / 8
+    threads = len(os.sched_getaffinity(0))
+    Q = O.schwarz(B, threads)
+    t0 = time.perf_counter()
+    J, K, nq = O.jk_screened(B, P, thr=1e-12, nthreads=threads, Q=Q)
+    t = time.perf_counter() - t0
+    log("  CPU restatement, %d waters: one J/K build %.1f s on %d threads (%.0f quartets)" % (nwater, t, threads, nq))
+    return {"waters": nwater, "nbf": B.nbf, "seconds": t, "builds_per_s": 1.0 / t, "threads": threads,
+            "quartets": nq}, (atno, xyz, P)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,8 +170,12 @@ def main():
                     help="timing density: converged FP64 SCF density (default) or the core-Hamiltonian guess")
     ap.add_argument("--fp32-threshold", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ref-waters", default="1,2",
+                    help="water counts of the reference Fortran timing (comma list, at most 4; '' skips it)")
+    ap.add_argument("--no-full-cpu-build", action="store_true",
+                    help="skip the complete 16-water build of the CPU restatement")
     ap.add_argument("--no-gradient", action="store_true", help="skip the analytic-gradient timing")
-    ap.add_argument("--forces", type=int, default=0, metavar="N",
+    ap.add_argument("--forces", type=int, default=10, metavar="N",
                     help="also run N consecutive energy+force evaluations (SCF warm-started from the previous "
                          "density) on seeded +-0.01 A perturbations, mixed engine vs FP64 engine (10 in the "
                          "mixed-precision force workload); reported under 'forces'")
@@ -217,16 +290,29 @@ def main():
         t0 = time.perf_counter()
         e64.grad_2e(Pc)
         t2 = time.perf_counter() - t0
+        gs64 = e64.grad_2e_stats()
         emx.grad_2e(Pc)
         t0 = time.perf_counter()
         emx.grad_2e(Pc)
         t2mx = time.perf_counter() - t0
+        gsmx = emx.grad_2e_stats()
         t0 = time.perf_counter()
         e64.grad_1e(Pc, W)
         t1 = time.perf_counter() - t0
         log("  gradient: two-electron %.3f s (FP64), %.3f s (mixed); one-electron %.3f s" % (t2, t2mx, t1))
+        g64 = gs64.flops_fp64 / gs64.jk_seconds
+        gmx = (gsmx.flops_fp64 + gsmx.flops_fp32) / gsmx.jk_seconds
         grad = {"grad_2e_seconds_fp64": t2, "grad_2e_seconds_mixed": t2mx, "grad_1e_seconds": t1,
-                "gradients_per_s_mixed": 1.0 / (t2mx + t1)}
+                "gradients_per_s_mixed": 1.0 / (t2mx + t1), "gradients_per_s_fp64": 1.0 / (t2 + t1),
+                "roofline": {"kernel": "grad_row_kernel (all derivative launches of one gradient)",
+                             "bound": "valu", "unit": "TFLOP/s", "peak": PEAK_VALU / 1e12,
+                             "flops_fp64_gradient": gs64.flops_fp64, "kernel_seconds_fp64": gs64.jk_seconds,
+                             "achieved_fp64": g64 / 1e12, "frac_fp64": g64 / PEAK_VALU,
+                             "flops_mixed_gradient": gsmx.flops_fp64 + gsmx.flops_fp32,
+                             "kernel_seconds_mixed": gsmx.jk_seconds, "achieved_mixed": gmx / 1e12,
+                             "frac_mixed": gmx / PEAK_VALU}}
+        log("  gradient kernels: FP64 %.3f s, %.1f TFLOP/s (%.3f of peak); mixed %.3f s, %.1f TFLOP/s"
+            % (gs64.jk_seconds, g64 / 1e12, g64 / PEAK_VALU, gsmx.jk_seconds, gmx / 1e12))
 
     forces = None
     if args.forces > 0:
@@ -265,19 +351,39 @@ def main():
                   "perturbation": "uniform +-0.01 A per coordinate about the base geometry, seed %d" % (args.seed + 1),
                   "incremental_threshold": args.incremental, "purification": bool(args.purify)}
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(e64, b, st64[0].quartets_fp64)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(e64, b, st64[0].quartets_fp64)
+        model, ncores = host_cpu()
+        cpu["host"] = {"cpu_model": model, "cores_available": ncores}
+        wl = [int(x) for x in args.ref_waters.split(",") if x.strip()]
+        if any(w > 4 for w in wl):
+            raise SystemExit("--ref-waters: the reference runs at most 4 waters (n <= 120)")
+        if wl:
+            cpu["reference_fortran"] = reference_cpu(wl)
+        if not args.no_full_cpu_build:
+            full, (atno16, xyz16, P16) = restatement_full_build()
+            # the GPU FP64 engine on the same 16-water build
+            e16 = Engine(Basis(atno16, xyz16))
+            e16.fock_2e(P16)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                e16.fock_2e(P16)
+            full["gpu_fp64_builds_per_s_host_buffers"] = 5.0 / (time.perf_counter() - t0)
+            e16.close()
+            cpu["restatement_full_build"] = full
 
     jk64 = np.mean([s.jk_seconds for s in st64])
     jkmx = np.mean([s.jk_seconds for s in stmx])
     f64 = st64[0].flops_fp64
     fmx64, fmx32 = stmx[0].flops_fp64, stmx[0].flops_fp32
     roof64 = f64 / jk64
-    # mixed: fraction of the time-weighted peak (FP64 part at 78.6, FP32 part at 157.3 TFLOP/s)
-    frac_mx = (fmx64 / PEAK_FP64 + fmx32 / PEAK_FP32) / jkmx
+    roofmx = (fmx64 + fmx32) / jkmx
     value = args.steps / tmx
-    traffic = None
-    if args.nwater == 64 and args.seed == 64:
-        traffic = measured_traffic()
+    traffic = traffic64 = None
+    if args.nwater == 64 and args.seed == 64 and abs(args.fp32_threshold - 1e-5) < 1e-20:
+        traffic = measured_traffic("traffic_bytes_per_mixed_build")
+        traffic64 = measured_traffic("traffic_bytes_per_fp64_build")
     nint = stmx[0].integrals_fp64 + stmx[0].integrals_fp32
     out = {
         "metric": "J+K Fock builds/s, 64 H2O 6-31G**: mixed-FP32 vs FP64 GPU, |dE|<1e-6 Eh",
@@ -309,12 +415,14 @@ def main():
         "E_elec_fp64_at_density": E64,
         "gradient": grad,
         "forces": forces,
-        "roofline": {"bound": "fp64-valu", "achieved": roof64 / 1e12, "peak": PEAK_FP64 / 1e12,
-                     "unit": "TFLOP/s", "frac": roof64 / PEAK_FP64, "traffic": traffic,
-                     "kernel": "jk_row_kernel/jk_kernel (all class launches of one FP64 build)",
-                     "jk_seconds_fp64": float(jk64), "jk_seconds_mixed": float(jkmx),
-                     "flops_fp64": float(f64), "traffic_source": TRAFFIC_FILE if traffic else None,
-                     "mixed_frac_of_weighted_peak": frac_mx},
+        "roofline": {"bound": "valu", "achieved": roofmx / 1e12, "peak": PEAK_VALU / 1e12,
+                     "unit": "TFLOP/s", "frac": roofmx / PEAK_VALU, "traffic": traffic,
+                     "kernel": "jk_row_kernel (all class launches of one mixed-precision build)",
+                     "jk_seconds_mixed": float(jkmx), "flops_mixed": float(fmx64 + fmx32),
+                     "flops_mixed_fp64_part": float(fmx64), "flops_mixed_fp32_part": float(fmx32),
+                     "fp64_build": {"achieved": roof64 / 1e12, "frac": roof64 / PEAK_FP64,
+                                    "jk_seconds": float(jk64), "flops": float(f64), "traffic": traffic64},
+                     "traffic_source": TRAFFIC_FILE if (traffic or traffic64) else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -62,6 +62,7 @@ struct hfx_engine {
     int device = 0;
     hipStream_t stream = nullptr;
     void* scratch = nullptr;   // device address of this GPU's allocation-free scratch array
+    hfx_stats grad_stats{};    // statistics of the last hfx_grad_2e call
     // basis
     int nbf = 0;
     int nat = 0;
@@ -395,6 +396,35 @@ FlopModel flop_model(int la, int lb, int lc, int ld) {
     per_root += 3.0 * NA * NB * NC * ND;       // Ix*Iy, fma with Iz
     f += NR * per_root;
     return FlopModel{f, 12.0};
+}
+
+// FLOPs of one primitive quartet / one contracted integral of the analytic
+// gradient (grad_rys.h, un-sliced; frozen model of the derivative kernel's
+// roofline figure): the energy's set-up and Rys rule for NR = (L + 1)/2 + 1
+// roots; per root the three directions' 2-D tables of the combined bra pass
+// (built one power higher on A, transferred to B one power higher) and of the
+// C pass (one power higher on C), and per integral the nine derivative
+// contractions (3 centres x 3 directions: a two-term derivative factor, 3,
+// and gam * dI_x * I_y * I_z, 4); per integral once the two-particle density
+// factor gam = s (4 D_ab D_cd - D_ac D_bd - D_ad D_bc), 7.
+FlopModel grad_flop_model(int la, int lb, int lc, int ld) {
+    const int NR = (la + lb + lc + ld + 1) / 2 + 1;
+    const double N = (double)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+    auto tables = [](int lab, int lcd, int lak, int lbk, int ldk) {
+        double g = (lab > 0 ? 1 : 0) + 3.0 * std::max(0, lab - 1);
+        for (int mm = 0; mm < lcd; mm++)
+            for (int nn = 0; nn <= lab; nn++) g += 1 + (mm > 0 ? 3 : 0) + (nn > 0 ? 3 : 0);
+        for (int ib = 1; ib <= lbk; ib++) g += 2.0 * (lab - ib + 1) * (lcd + 1);
+        for (int id = 1; id <= ldk; id++) g += 2.0 * (lak + 1) * (lbk + 1) * (lcd - id + 1);
+        return g;
+    };
+    double f = 30 + 2.0 * NR * (3.0 * 13 + 3.0) + 8;
+    double per_root = 16;
+    per_root += 3 * tables(la + lb + 1, lc + ld, la + 1, lb + 1, ld);   // A and B pass
+    per_root += 3 * tables(la + lb, lc + ld + 1, la, lb, ld);           // C pass
+    per_root += 9.0 * 7.0 * N;
+    f += NR * per_root;
+    return FlopModel{f, 7.0};
 }
 
 // ------------------------------------------------------------- screening ---
@@ -1161,9 +1191,11 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     const int n = h->nbf;
     const size_t nn = (size_t)n * n;
     if (ensure_work(h)) return 1;
+    const auto t0 = std::chrono::steady_clock::now();
     HIPCHK(h, hipMemcpy(h->d_P, P, nn * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(h, hfx_launch_prep(h->d_P, h->d_scale, h->d_D, h->d_Jb, h->d_Kb, n, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_grad, 0, 3 * (size_t)h->nat * sizeof(double), h->stream));
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     // row mode over the ket tables where the J/K launch is in row mode
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
         if (h->ktab[c].t.nk > 0)
@@ -1249,8 +1281,34 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
             HIPCHK(h, hfx_launch_grad(c.cb, c.ck, prec == 1, a, c.gnblk[prec], h->stream));
         }
     }
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(g, h->d_grad, 3 * (size_t)h->nat * sizeof(double), hipMemcpyDeviceToHost));
+    // statistics: the same quartets as the Fock build, priced by the derivative model
+    hfx_stats st{};
+    for (const auto& c : h->combos) {
+        const int la = pair_class_la(c.cb), lb = pair_class_lb(c.cb), lc = pair_class_la(c.ck),
+                  ld = pair_class_lb(c.ck);
+        const FlopModel fm = grad_flop_model(la, lb, lc, ld);
+        const double nint = (double)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+        for (int prec = 0; prec < 2; prec++) {
+            const double fl = c.nprimq[prec] * fm.per_prim + c.nq[prec] * nint * fm.per_int;
+            (prec == 0 ? st.quartets_fp64 : st.quartets_fp32) += c.nq[prec];
+            (prec == 0 ? st.flops_fp64 : st.flops_fp32) += fl;
+            (prec == 0 ? st.integrals_fp64 : st.integrals_fp32) += c.nq[prec] * nint;
+        }
+    }
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    st.jk_seconds = 1e-3 * ms;
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    h->grad_stats = st;
+    return 0;
+}
+
+int hfx_grad_2e_stats(const hfx_t* h, hfx_stats* st) {
+    if (!h || !st) return 1;
+    *st = h->grad_stats;
     return 0;
 }
 

@@ -43,6 +43,7 @@ SIGNATURES = {
     "hfx_fock_2e_device": (_I, [_P, _P, _P, _P, _P, ctypes.POINTER(HfxStats)]),
     "hfx_one_electron": (_I, [_P, _P, _P, _P]),
     "hfx_grad_2e": (_I, [_P, _P, _P]),
+    "hfx_grad_2e_stats": (_I, [_P, ctypes.POINTER(HfxStats)]),
     "hfx_grad_1e": (_I, [_P, _P, _P, _P]),
     "hfx_set_geometry": (_I, [_P, _P]),
     "hfx_eri_shells": (_I, [_P, _I, _P, _P]),
@@ -170,6 +171,12 @@ class Engine:
         g = np.zeros(3 * self.nat)
         self._check(lib().hfx_grad_2e(self._h, _ptr(P), _ptr(g)))
         return g.reshape(self.nat, 3)
+
+    def grad_2e_stats(self):
+        """Statistics of the last grad_2e call (HfxStats: time, quartets, model FLOPs, kernel seconds)."""
+        st = HfxStats()
+        self._check(lib().hfx_grad_2e_stats(self._h, ctypes.byref(st)))
+        return st
 
     def grad_1e(self, P, W):
         P = np.asfortranarray(P, dtype=np.float64)

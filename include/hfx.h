@@ -105,6 +105,15 @@ int hfx_one_electron(hfx_t *h, double *S, double *T, double *V);
 int hfx_grad_2e(hfx_t *h, const double *P, double *g);
 
 /*
+ * Statistics of the last hfx_grad_2e call on h (zeros before the first):
+ * seconds (wall time), the shell quartets and contracted integrals it
+ * differentiated (FP64 / FP32 launches), their FLOPs by the frozen operation
+ * count of the derivative kernel (flops_fp64 / flops_fp32), and in jk_seconds
+ * the derivative kernels' time by HIP events on the engine's stream.
+ */
+int hfx_grad_2e_stats(const hfx_t *h, hfx_stats *st);
+
+/*
  * One-electron part of the RHF gradient:
  *   g_X = sum_ij P_ij dH_ij/dX - sum_ij W_ij dS_ij/dX  (Hellmann-Feynman term included),
  * W = energy-weighted density. g: 3*nat. Evaluated on the GPU (one_e_kernel).
