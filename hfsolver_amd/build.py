@@ -5,6 +5,8 @@
 
 The kernels (csrc/hfx_kernels.hip) are compiled for gfx950 only, with packed
 FP32 instructions disabled; the host runtime (csrc/hfx_host.cpp) with OpenMP.
+The Cython binding (_hfx.pyx over c_hfx.pxd) is compiled against the library
+into hfsolver_amd/_hfx<EXT_SUFFIX>.
 """
 import os
 import subprocess
@@ -58,8 +60,26 @@ def build_library(verbose=True):
     return LIB
 
 
+def build_cython(verbose=True):
+    """Cython -> C (_build/_hfx.c) -> hfsolver_amd/_hfx<EXT_SUFFIX>, linked to libhfx.so."""
+    import sysconfig
+    os.makedirs(BUILD, exist_ok=True)
+    csrc = os.path.join(BUILD, "_hfx.c")
+    run([sys.executable, "-m", "cython", "-3", "--module-name", "hfsolver_amd._hfx", "-I", HERE,
+         "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "_hfx.pyx"), "-o", csrc])
+    ext = os.path.join(HERE, "_hfx" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = ext + ".tmp"
+    run(["gcc", "-O2", "-fPIC", "-shared", "-I" + sysconfig.get_paths()["include"], "-I" + os.path.join(ROOT, "include"),
+         csrc, "-L" + HERE, "-lhfx", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, ext)
+    if verbose:
+        print("built", ext)
+    return ext
+
+
 def main(argv):
     build_library()
+    build_cython()
     return 0
 
 

@@ -66,27 +66,67 @@ class ForceField:
         self.engine.close()
 
 
+def make_engine(basis, binding="cython", **kw):
+    """An engine for basis through the Cython binding (hfsolver_amd.hfx, the
+    default) or the ctypes binding (hfsolver_amd.engine); both wrap the same C ABI."""
+    if binding == "cython":
+        from . import hfx
+        return hfx.Engine(basis, **kw)
+    if binding == "ctypes":
+        return Engine(basis, **kw)
+    raise ValueError("binding must be 'cython' or 'ctypes'")
+
+
+# statistics of the derivative kernels run by the last RHFEnergy.backward
+last_backward_stats = None
+
+
 def _torch_function():
     import torch
 
     class RHFEnergy(torch.autograd.Function):
-        """E_tot(R) of closed-shell RHF; backward = analytic nuclear gradient."""
+        """E_tot(R) of closed-shell RHF. forward runs the SCF and keeps the engine
+        and the converged orbitals; backward runs the derivative kernels
+        (hfx_grad_2e, hfx_grad_1e) with P and W of that SCF."""
 
         @staticmethod
         def forward(ctx, xyz, atno, opts):
-            x = xyz.detach().to("cpu", torch.float64).numpy()
-            r = rhf_gauss(atno, x, keep_engine=True, **opts)
+            x = xyz.detach().to("cpu", torch.float64).numpy().reshape(-1, 3)
+            opts = dict(opts)
+            binding = opts.pop("binding", "cython")
+            eng = opts.pop("engine", None)
+            own = eng is None
+            if own:
+                kw = {k: opts.pop(k) for k in ("schwarz_threshold", "fp32_threshold", "device") if k in opts}
+                eng = make_engine(Basis(atno, x), binding, **kw)
+            elif not np.array_equal(x, eng.basis.xyz):
+                eng.set_geometry(x)
             try:
-                g = rhf_gradient(r)
-            finally:
-                r.engine.close()
-            ctx.save_for_backward(torch.from_numpy(g).to(dtype=xyz.dtype, device=xyz.device))
+                r = rhf_gauss(atno, x, engine=eng, **opts)
+            except Exception:
+                if own:
+                    eng.close()
+                raise
+            ctx.engine, ctx.own, ctx.result = eng, own, r
             return torch.tensor(r.Etot, dtype=xyz.dtype, device=xyz.device)
 
         @staticmethod
         def backward(ctx, gout):
-            (g,) = ctx.saved_tensors
-            return gout * g, None, None
+            global last_backward_stats
+            eng, r = ctx.engine, ctx.result
+            try:
+                atno = eng.basis.atno
+                nocc = int(np.sum(atno)) // 2
+                P = 2.0 * r.C[:, :nocc] @ r.C[:, :nocc].T
+                W = energy_weighted_density(r.C, r.lam, nocc)
+                g = eng.grad_2e(P)
+                last_backward_stats = eng.grad_2e_stats()
+                g = g + eng.grad_1e(P, W) + nuclear_gradient(atno, eng.basis.xyz)
+            finally:
+                if ctx.own:
+                    eng.close()
+                ctx.engine = ctx.result = None
+            return gout * torch.from_numpy(g).to(dtype=gout.dtype, device=gout.device), None, None
 
     return RHFEnergy
 
@@ -96,7 +136,10 @@ _RHF = None
 
 def rhf_energy(xyz, atno, **opts):
     """Differentiable total energy: xyz is a float64 torch tensor (nat, 3) in bohr;
-    opts are rhf_gauss keyword arguments (Nscf, alpha, tolE, tolP, diis, fp32_threshold, ...)."""
+    opts are rhf_gauss keyword arguments (Nscf, alpha, tolE, tolP, diis, P0,
+    eig, fp32_threshold, schwarz_threshold, ...), plus binding="cython" (the
+    default) or "ctypes", and engine= an existing engine to reuse (moved to
+    xyz; not closed)."""
     global _RHF
     if _RHF is None:
         _RHF = _torch_function()

@@ -57,3 +57,27 @@ def test_bad_basis_is_rejected_before_device_work():
     b.l[9] = 0  # first d component of O no longer x^2
     with pytest.raises(E.HfxError):
         E.Engine(b)
+
+
+def test_cython_declarations_cover_the_abi():
+    """c_hfx.pxd (the Cython layer, as hfsolver/c_hfsolver.pxd) declares every
+    entry point of include/hfx.h."""
+    src = open(os.path.join(ROOT, "hfsolver_amd", "c_hfx.pxd")).read()
+    pxd = sorted(set(re.findall(r"\b(hfx_[a-z0-9_]+)\s*\(", src)))
+    assert pxd == declared()
+
+
+def test_cython_module_loads_and_fails_loudly_without_gpu():
+    from hfsolver_amd import hfx
+    names = ["fock_2e", "fock_2e_device", "one_electron", "grad_2e", "grad_2e_stats", "grad_1e", "set_geometry",
+             "eri_shells", "int2", "shells", "pair_bounds", "set_density_screening", "screening", "close"]
+    for n in names:
+        assert hasattr(hfx.Engine, n) and hasattr(E.Engine, n), n
+    b = Basis([8, 1, 1], [[0, 0, 0], [1.809, 0, 0], [-0.453549, 1.751221, 0]])
+    try:
+        eng = hfx.Engine(b)
+    except hfx.HfxError as e:
+        assert "device" in str(e)
+        return
+    eng.close()
+    pytest.skip("a HIP device is present; covered by the gpu tests")
