@@ -7,11 +7,16 @@
 ! (scalars by value, arrays by reference); a non-zero status is turned into
 ! stop_error semantics (message, exit code 1) as src/utils.f90:171-186 does.
 module hfx_gpu
-use iso_c_binding, only: c_int, c_double, c_ptr, c_null_ptr, c_char, c_associated, c_f_pointer
+use iso_c_binding, only: c_int, c_double, c_long_long, c_ptr, c_null_ptr, c_char, c_associated, c_f_pointer
 implicit none
 private
 public hfx_opts, hfx_default_opts, hfx_create, hfx_fock_2e, hfx_one_electron, hfx_destroy, &
-    hfx_last_error, hfx_init, hfx_formG
+    hfx_last_error, hfx_init, hfx_formG, hfx_store_int2, hfx_direct
+
+! The engine of the current basis in direct mode: created (and replaced for
+! every new basis) by getints2 of the direct `gaussians` module
+! (gaussians_direct.f90), used by the direct `doscf` (scf_direct.f90).
+type(c_ptr), save :: hfx_direct = c_null_ptr
 
 type, bind(c) :: hfx_opts
     real(c_double) :: schwarz_threshold
@@ -51,6 +56,13 @@ interface
     real(c_double), intent(out) :: S(*), T(*), V(*)
     end function
 
+    integer(c_int) function hfx_int2_c(h, int2, size) bind(c, name="hfx_int2")
+    import :: c_int, c_double, c_long_long, c_ptr
+    type(c_ptr), value, intent(in) :: h
+    real(c_double), intent(out) :: int2(*)
+    integer(c_long_long), value, intent(in) :: size
+    end function
+
     type(c_ptr) function hfx_last_error_c(h) bind(c, name="hfx_last_error")
     import :: c_ptr
     type(c_ptr), value, intent(in) :: h
@@ -87,18 +99,31 @@ end subroutine
 
 ! Engine for the per-function basis arrays of gaussints (src/basis.f90:541-567);
 ! istart is 1-based here, as gaussints holds it, and passed 0-based as getInts does.
-subroutine hfx_init(nprim, istart, center, power, coef, alpha, atno, xyz, h, fp32_threshold)
+subroutine hfx_init(nprim, istart, center, power, coef, alpha, atno, xyz, h, fp32_threshold, schwarz_threshold)
 integer, intent(in) :: nprim(:), istart(:), power(:, :), atno(:)
 real(c_double), intent(in) :: center(:, :), coef(:), alpha(:), xyz(:, :)
 type(c_ptr), intent(out) :: h
-real(c_double), intent(in), optional :: fp32_threshold
+real(c_double), intent(in), optional :: fp32_threshold, schwarz_threshold
 type(hfx_opts) :: o
 integer(c_int) :: rc
 rc = hfx_default_opts(o)
 if (present(fp32_threshold)) o%fp32_threshold = fp32_threshold
+if (present(schwarz_threshold)) o%schwarz_threshold = schwarz_threshold
 rc = hfx_create(size(nprim), nprim, istart - 1, center(1, :), center(2, :), center(3, :), &
     power(1, :), power(2, :), power(3, :), size(coef), coef, alpha, size(atno), atno, xyz, o, h)
 if (rc /= 0) call hfx_fail(h, "hfx_create")
+end subroutine
+
+! Every unique (ij|kl) in the packed order of ijkl2intindex (store mode,
+! hfx_int2), from an engine of its own that keeps every primitive pair.
+subroutine hfx_store_int2(nprim, istart, center, power, coef, alpha, atno, xyz, int2)
+integer, intent(in) :: nprim(:), istart(:), power(:, :), atno(:)
+real(c_double), intent(in) :: center(:, :), coef(:), alpha(:), xyz(:, :)
+real(c_double), intent(out) :: int2(:)
+type(c_ptr) :: h
+call hfx_init(nprim, istart, center, power, coef, alpha, atno, xyz, h, schwarz_threshold=0.0_c_double)
+if (hfx_int2_c(h, int2, int(size(int2), c_long_long)) /= 0) call hfx_fail(h, "hfx_int2")
+call hfx_destroy(h)
 end subroutine
 
 ! G(P) = J - K/2: the drop-in for G = formG(P, TT) (src/scf.f90:26-37).

@@ -22,6 +22,8 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 MOLTEST = os.path.join(ROOT, "oracle", "_ref", "test_molecules_gpu")
+MOLTEST_DIRECT = {"make": os.path.join(ROOT, "oracle", "_ref", "test_molecules_direct"),
+                  "cmake": os.path.join(ROOT, "oracle", "_ref", "cmake_direct", "test_molecules_direct")}
 
 
 def say(*a):
@@ -76,6 +78,30 @@ def test_reference_molecule_tests_with_gpu_getints2(tmp_path):
     tail = "\n".join(r.stdout.splitlines()[-12:])
     say(tail)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "E2    (MBPT2)" in r.stdout
+
+
+@pytest.mark.parametrize("build", ["make", "cmake"])
+def test_reference_molecule_tests_with_direct_jk(tmp_path, build):
+    """The reference's test_gaussians_molecules in direct mode: its unchanged
+    drivers.f90 / basis.f90 call the direct gaussians module (engine kept,
+    int2 still filled for MBPT) and the direct scf module, whose doscf takes
+    G(P) from hfx_fock_2e every iteration instead of building TT. The program's
+    own checks (SCF energies, HOMOs, MBPT2/3, Green's-function poles against
+    known answers) must pass. Built by oracle/Makefile.ref directly and through
+    the WITH_HIP CMake fragment (hfsolver_amd/fortran/hfx_hip.cmake)."""
+    exe = MOLTEST_DIRECT[build]
+    if not os.path.exists(exe):
+        pytest.skip("direct-mode molecule test not built (oracle/Makefile.ref)")
+    say("reference test_gaussians_molecules in direct mode (G from hfx_fock_2e), %s build" % build)
+    shutil.copy(os.path.join(ROOT, "hfsolver_amd", "data", "p631ss.txt"), tmp_path / "p631ss.txt")
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited && exec %s" % exe], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=600)
+    say("\n".join(r.stdout.splitlines()[-8:]))
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    nscf = r.stdout.count("doscf: G(P) from the direct J/K engine")
+    assert nscf >= 8                                   # every rhf_gauss of the program
+    assert "Precalculating J-0.5*K" not in r.stdout    # the reference's TT build never ran
     assert "E2    (MBPT2)" in r.stdout
 
 
