@@ -64,7 +64,7 @@ cdef class Engine:
         o.schwarz_threshold = schwarz_threshold
         o.fp32_threshold = fp32_threshold
         o.device = device
-        o.verbose = 1 if verbose else 0
+        o.verbose = int(verbose)
         o.kernel_mode = kernel_mode
         b = basis
         cdef int[::1] nprim = _i32(b.nprim), istart = _i32(b.istart), l = _i32(b.l), m = _i32(b.m), nn = _i32(b.n)

@@ -54,17 +54,19 @@ DEGF = 6          # single-precision copy: the series truncated after degree 6
 DPS = 60
 # Boys-function Taylor table for the one- and two-root rules (closed form from
 # F_0..F_3): F_m(T) = sum_{k<BOYS_K} F_{m+k}(T0) (T0-T)^k / k!, T0 on a grid of
-# step BOYS_DT, |T - T0| <= BOYS_DT/2, error < (1/64)^7/7! ~ 5e-17 relative
-# (grid 1/8 with 9 terms has the same accuracy; the denser grid needs fewer
-# loads and multiply-adds per evaluation).
+# step BOYS_DT = 1/32, |T - T0| <= 1/64, truncation < (1/64)^7/7! ~ 5e-17
+# relative (grid 1/8 with 9 terms has the same accuracy; the denser grid needs
+# fewer loads and multiply-adds per evaluation). Per grid point the table
+# holds F_0 .. F_{BOYS_MMAX + BOYS_K - 1} = F_0 .. F_9.
 BOYS_DT = 0.03125
 BOYS_TMAX = 46.0
 BOYS_K = 7
 BOYS_MMAX = 3                      # highest order needed (two roots: F_0..F_3)
-BOYS_NCOL = BOYS_MMAX + BOYS_K     # F_0 .. F_11 per grid point
+BOYS_NCOL = BOYS_MMAX + BOYS_K     # F_0 .. F_9 per grid point
 BOYS_NPT = int(BOYS_TMAX / BOYS_DT) + 1
-# single-precision copy on the same grid for the FP32 kernels: BOYSF_K terms,
-# error < (1/16)^5/5! ~ 8e-9 relative, below float rounding
+# single-precision copy on the same grid for the FP32 kernels (used only by a
+# build with HFX_FP32_RULES=1): BOYSF_K terms, truncation < (1/64)^5/5! ~ 8e-12
+# relative, below float rounding
 BOYSF_K = 5
 BOYSF_NCOL = BOYS_MMAX + BOYSF_K   # F_0 .. F_7 per grid point
 

@@ -194,10 +194,28 @@ int dmalloc(hfx_t* h, void** p, size_t bytes, bool geom) {
     return 0;
 }
 
+// Device buffer of one call (not owned by the handle), freed on every return path.
+struct DeviceTemp {
+    void* p = nullptr;
+    DeviceTemp() = default;
+    DeviceTemp(const DeviceTemp&) = delete;
+    DeviceTemp& operator=(const DeviceTemp&) = delete;
+    ~DeviceTemp() {
+        if (p) (void)hipFree(p);
+    }
+};
+
 template <typename T>
 int upload(hfx_t* h, T** dptr, const std::vector<T>& v, bool geom) {
     if (dmalloc(h, (void**)dptr, v.size() * sizeof(T), geom)) return 1;
     if (!v.empty()) HIPCHK(h, hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+int upload_temp(hfx_t* h, DeviceTemp& b, const std::vector<T>& v) {
+    if (dmalloc(h, &b.p, v.size() * sizeof(T), false)) return 1;
+    if (!v.empty()) HIPCHK(h, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1063,6 +1081,8 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     a.K = h->d_Kb;
     a.n = n;
     double q64 = 0, q32 = 0, f64 = 0, f32 = 0, i64 = 0, i32 = 0;
+    double cls_fl[HFX_NPAIRCLASS][HFX_NPAIRCLASS][2] = {}, cls_nq[HFX_NPAIRCLASS][HFX_NPAIRCLASS][2] = {},
+           cls_np[HFX_NPAIRCLASS][HFX_NPAIRCLASS][2] = {};
     for (const auto& c : h->combos) {
         a.bra_list = h->d_cls_list[c.cb];
         a.ket_list = c.d_ket;
@@ -1081,6 +1101,9 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
                 HIPCHK(h, hfx_launch_jk(c.cb, c.ck, prec == 1, a, c.ntiles[prec], h->stream));
             }
             const double fl = c.nprimq[prec] * fm.per_prim + c.nq[prec] * nint * fm.per_int;
+            cls_fl[c.cb][c.ck][prec] += fl;
+            cls_nq[c.cb][c.ck][prec] += c.nq[prec];
+            cls_np[c.cb][c.ck][prec] += c.nprimq[prec];
             if (prec == 0) {
                 q64 += c.nq[0];
                 f64 += fl;
@@ -1100,6 +1123,15 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     HIPCHK(h, hipStreamSynchronize(h->stream));
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (h->opts.verbose >= 2)   // per class combination and precision: the roofline's FLOP model
+        for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
+            for (int ck = 0; ck <= cb; ck++)
+                for (int prec = 0; prec < 2; prec++)
+                    if (cls_nq[cb][ck][prec] > 0)
+                        fprintf(stderr, "hfx: build (%d%d|%d%d) %s quartets %.6g prim_quartets %.6g flops %.6g\n",
+                                pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck),
+                                prec ? "fp32" : "fp64", cls_nq[cb][ck][prec], cls_np[cb][ck][prec],
+                                cls_fl[cb][ck][prec]);
     if (st) {
         st->jk_seconds = 1e-3 * ms;
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1133,7 +1165,8 @@ int hfx_one_electron(hfx_t* h, double* S, double* T, double* V) {
     HIPCHK(h, hipSetDevice(h->device));
     if (ensure_work(h)) return 1;
     const size_t nn = (size_t)h->nbf * h->nbf;
-    // shell pairs whose every primitive pair is screened out keep zeros
+    // cleared first; the one-electron lists hold every shell pair (no screening),
+    // whose wavefront then stores all elements of its blocks
     HIPCHK(h, hipMemsetAsync(h->d_G, 0, nn * sizeof(double), h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_J, 0, nn * sizeof(double), h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_K, 0, nn * sizeof(double), h->stream));
@@ -1269,6 +1302,9 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     a.n = n;
     for (const auto& c : h->combos) {
         if (done[c.cb][c.ck]) continue;
+        if (3 * (size_t)h->nat * sizeof(double) > HFX_GRAD_LDS_MAX)
+            return fail(h, "hfx_grad_2e: %d atoms need %zu bytes of LDS in the fallback gradient kernel (at most %zu)",
+                        h->nat, 3 * (size_t)h->nat * sizeof(double), HFX_GRAD_LDS_MAX);
         a.bra_list = h->d_cls_list[c.cb];
         a.ket_list = c.d_ket;
         for (int prec = 0; prec < 2; prec++) {
@@ -1346,11 +1382,12 @@ int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
         tot += (long long)ncart(h->shells[A].L) * ncart(h->shells[B].L) * ncart(h->shells[C].L) *
                ncart(h->shells[D].L);
     }
-    int *d_bra = nullptr, *d_ket = nullptr, *d_q = nullptr;
-    long long* d_off = nullptr;
-    double* d_out = nullptr;
-    if (upload(h, &d_bra, bra, false) || upload(h, &d_ket, ket, false) || upload(h, &d_off, off, false)) return 1;
-    if (dmalloc(h, (void**)&d_out, tot * sizeof(double), false)) return 1;
+    DeviceTemp b_bra, b_ket, b_off, b_out, b_q;
+    if (upload_temp(h, b_bra, bra) || upload_temp(h, b_ket, ket) || upload_temp(h, b_off, off)) return 1;
+    if (dmalloc(h, &b_out.p, tot * sizeof(double), false)) return 1;
+    int *d_bra = (int*)b_bra.p, *d_ket = (int*)b_ket.p;
+    long long* d_off = (long long*)b_off.p;
+    double* d_out = (double*)b_out.p;
     // group by class combination
     std::vector<int> order(nq);
     for (int q = 0; q < nq; q++) order[q] = q;
@@ -1358,7 +1395,8 @@ int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
         const int kx = h->pairs[bra[x]].cls * 8 + h->pairs[ket[x]].cls, ky = h->pairs[bra[y]].cls * 8 + h->pairs[ket[y]].cls;
         return kx < ky;
     });
-    if (upload(h, &d_q, order, false)) return 1;
+    if (upload_temp(h, b_q, order)) return 1;
+    const int* d_q = (const int*)b_q.p;
     int s = 0;
     while (s < nq) {
         const int cb = h->pairs[bra[order[s]]].cls, ck = h->pairs[ket[order[s]]].cls;
@@ -1381,11 +1419,6 @@ int hfx_eri_shells(hfx_t* h, int nq, const int* shells, double* out) {
     std::vector<double> tmp(tot);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(tmp.data(), d_out, tot * sizeof(double), hipMemcpyDeviceToHost));
-    (void)hipFree(d_bra);
-    (void)hipFree(d_ket);
-    (void)hipFree(d_off);
-    (void)hipFree(d_out);
-    (void)hipFree(d_q);
     // undo the orientation: kernel layout is (a' b' | c' d') of the oriented pairs
     for (int q = 0; q < nq; q++) {
         const int A = shells[4 * q], B = shells[4 * q + 1], C = shells[4 * q + 2], D = shells[4 * q + 3];
@@ -1572,12 +1605,7 @@ int hfx_transform_int2(hfx_t* h, const double* int2, const double* C, double* mo
         if (rocblas_create_handle(&h->blas) != rocblas_status_success) return fail(h, "rocblas_create_handle failed");
     }
     if (rocblas_set_stream(h->blas, h->stream) != rocblas_status_success) return fail(h, "rocblas_set_stream failed");
-    struct Scoped {   // device buffers of this call, freed on every return path
-        void* p = nullptr;
-        ~Scoped() {
-            if (p) (void)hipFree(p);
-        }
-    } bX, bY, bP, bC;
+    DeviceTemp bX, bY, bP, bC;
     if (dmalloc(h, &bX.p, n4 * sizeof(double), false) || dmalloc(h, &bY.p, n4 * sizeof(double), false) ||
         dmalloc(h, &bP.p, (size_t)size * sizeof(double), false) ||
         dmalloc(h, &bC.p, (size_t)n * n * sizeof(double), false))
@@ -1643,10 +1671,11 @@ int hfx_int2(hfx_t* h, double* int2, long long size) {
         std::vector<int> fill(start.begin(), start.end() - 1);
         for (long long q = 0; q < nq; q++) order[fill[h->pairs[bra[q]].cls * 8 + h->pairs[ket[q]].cls]++] = (int)q;
     }
-    int *d_bra = nullptr, *d_ket = nullptr, *d_q = nullptr;
-    double* d_out = nullptr;
-    if (upload(h, &d_bra, bra, false) || upload(h, &d_ket, ket, false) || upload(h, &d_q, order, false)) return 1;
-    if (dmalloc(h, (void**)&d_out, (size_t)size * sizeof(double), false)) return 1;
+    DeviceTemp b_bra, b_ket, b_q, b_out;
+    if (upload_temp(h, b_bra, bra) || upload_temp(h, b_ket, ket) || upload_temp(h, b_q, order)) return 1;
+    if (dmalloc(h, &b_out.p, (size_t)size * sizeof(double), false)) return 1;
+    const int *d_bra = (const int*)b_bra.p, *d_ket = (const int*)b_ket.p, *d_q = (const int*)b_q.p;
+    double* d_out = (double*)b_out.p;
     HIPCHK(h, hipMemsetAsync(d_out, 0, (size_t)size * sizeof(double), h->stream));
     for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
         for (int ck = 0; ck <= cb; ck++) {
@@ -1667,10 +1696,6 @@ int hfx_int2(hfx_t* h, double* int2, long long size) {
         }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(int2, d_out, (size_t)size * sizeof(double), hipMemcpyDeviceToHost));
-    (void)hipFree(d_bra);
-    (void)hipFree(d_ket);
-    (void)hipFree(d_q);
-    (void)hipFree(d_out);
     return 0;
 }
 

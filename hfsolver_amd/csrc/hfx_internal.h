@@ -242,6 +242,8 @@ inline size_t hfx_grad_row_lds(int nrow, int ncols, int R, int dlds, bool fp32, 
 hipError_t hfx_launch_grad_rows(int cls_bra, int cls_ket, bool fp32, const GradRowArgs& a, int nblocks,
                                 hipStream_t s);
 hipError_t hfx_launch_block(int cls_bra, int cls_ket, const BlockArgs& a, hipStream_t s);
+// LDS limit of the fallback gradient kernel (3 nat doubles of per-atom sums)
+static constexpr size_t HFX_GRAD_LDS_MAX = 128 * 1024;
 // Device scratch of the allocation-free entry point: a file-scope array of
 // HFX_SCRATCH_BYTES per GPU. hfx_scratch_address returns its device address
 // for `device`, which must be the current device (looked up once per device).

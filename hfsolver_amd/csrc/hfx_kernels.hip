@@ -73,14 +73,8 @@ template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
     constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
-    // one- and two-root classes: Boys table staged in LDS for the closed-form rules
+    // the Boys table is read through the caches, as in the row-mode kernels
     const double* boys = hfx_dev::BOYS_TAB;
-    if constexpr (Q::NR <= 2) {
-        __shared__ double boys_lds[BOYS_SIZE];
-        for (int k = threadIdx.x; k < BOYS_SIZE; k += blockDim.x) boys_lds[k] = hfx_dev::BOYS_TAB[k];
-        __syncthreads();
-        boys = boys_lds;
-    }
     const int t = blockIdx.x;
     const int row = a.tile_i0[t] + (int)(threadIdx.x / HFX_TILE_COLS);
     const int col = a.tile_j0[t] + (int)(threadIdx.x % HFX_TILE_COLS);
@@ -567,15 +561,10 @@ template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
     constexpr bool SPA = gsplit_a(LA, LB, LC, LD), SPC = gsplit_c(LA, LB, LC, LD);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
-    constexpr int NRG = (LA + LB + LC + LD + 1) / 2 + 1;
     extern __shared__ double smem[];
     double* gl = smem;  // 3 * nat
+    // the Boys table is read through the caches, as in the row-mode kernels
     const double* boys = hfx_dev::BOYS_TAB;
-    if constexpr (NRG <= 2) {
-        double* bl = smem + 3 * a.nat;
-        for (int k = threadIdx.x; k < BOYS_SIZE; k += blockDim.x) bl[k] = hfx_dev::BOYS_TAB[k];
-        boys = bl;
-    }
     for (int k = threadIdx.x; k < 3 * a.nat; k += blockDim.x) gl[k] = 0.0;
     __syncthreads();
     const int row = a.blk_row[blockIdx.x];
@@ -818,12 +807,12 @@ const rows_fn ROWS_TABLE[6][6] = HFX_TABLE(rows_xy);
 typedef HFX_REAL_T GradReal;
 template <int X, int Y>
 hipError_t grad_xy(const GradArgs& a, int nblocks, hipStream_t s) {
-    constexpr int NRG = (pair_class_la(X) + pair_class_lb(X) + pair_class_la(Y) + pair_class_lb(Y) + 1) / 2 + 1;
-    const size_t lds = (3 * (size_t)a.nat + (NRG <= 2 ? BOYS_SIZE : 0)) * sizeof(double);
+    const size_t lds = 3 * (size_t)a.nat * sizeof(double);
+    if (lds > HFX_GRAD_LDS_MAX) return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void*)grad_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 32 * 1024);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)HFX_GRAD_LDS_MAX);
         if (e != hipSuccess) return e;
         attr_set = true;
     }

@@ -1,10 +1,14 @@
-// Rys quadrature nodes and weights from the generated piecewise-Chebyshev
+// Rys quadrature nodes and weights from the generated piecewise-polynomial
 // tables (rys_tables.inc, written by gen_rys_tables.py).
 //
-// For T in [0, RYS_TMAX): interval it = floor(T / RYS_H), x = 2 (T - it H)/H - 1,
-// and each u_i(T), w_i(T) is a Chebyshev series of degree RYS_DEG in x,
-// evaluated by Clenshaw's recurrence. For T >= RYS_TMAX the rule is
-// half-range Gauss-Hermite: u_i = x_i^2 / T, w_i = W_i / sqrt(T).
+// Three to five roots: for T in [0, RYS_TMAX) the interval is
+// it = floor(T / RYS_H), x = 2 (T - it H)/H - 1, and each u_i(T), w_i(T) is a
+// polynomial of degree RYS_DEG in x, stored in powers of x (converted from a
+// Chebyshev interpolant at high precision) and evaluated by Horner's rule; the
+// single-precision tables of the FP32 kernels keep a truncated Chebyshev
+// series. One and two roots: closed forms from the Boys moments F_0..F_3 of a
+// Taylor table (BOYS_TAB: a 1/32 grid, BOYS_K terms). For T >= RYS_TMAX the
+// rule is half-range Gauss-Hermite: u_i = x_i^2 / T, w_i = W_i / sqrt(T).
 // sum_i w_i = F_0(T), the Boys function of hfsolver's src/special_functions.f90:189-248.
 #pragma once
 
@@ -16,7 +20,7 @@ struct RysTables {
     const double* poly;  // RYS_POLY
     const double* hu;    // RYS_HERM_U
     const double* hw;    // RYS_HERM_W
-    const double* boys;  // BOYS_TAB (global memory, or a copy in LDS)
+    const double* boys;  // BOYS_TAB (global memory, read through the caches)
     const float* boysf;  // BOYSF_TAB, single-precision Boys table of the FP32 kernels
     const float* chebf;  // RYS_CHEBF, single-precision truncated Chebyshev series of the FP32 kernels
 };

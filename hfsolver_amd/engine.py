@@ -102,7 +102,7 @@ class Engine:
         opts.schwarz_threshold = schwarz_threshold
         opts.fp32_threshold = fp32_threshold
         opts.device = device
-        opts.verbose = 1 if verbose else 0
+        opts.verbose = int(verbose)
         opts.kernel_mode = kernel_mode
         h = ctypes.c_void_p()
         b = basis

@@ -142,3 +142,25 @@ def test_derivative_quartets_match_oracle(lib):
             assert lib.chk_grad_quartet(*args, p(gs), fp32, p(g)) == 0
             assert np.abs(g.reshape(3, 3) - ref[:3]).max() <= 1e-14 + tol * m, (key, fp32)
     assert len(seen) == 21
+
+
+def test_rys_rules_at_the_worst_table_offsets(lib):
+    """Every interval of the tables at its worst point. One and two roots: the
+    Boys Taylor table at T = (k + 1/2)/32 - 1e-12, the largest distance 1/64
+    from a grid point, k over all 1472 points of [0, 46); three to five roots:
+    the midpoints of all 480 polynomial intervals (width 1/8) of [0, 60). Moments
+    against the oracle's Boys function; the zeroth within 8e-16 F_0, which a
+    Taylor series one term short (about 1.6e-15 F_0 near T = 0) would miss."""
+    lib.chk_rys.restype = ctypes.c_double
+    u = np.zeros(5)
+    w = np.zeros(5)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    cases = [(nr, -nr, [(k + 0.5) / 32 - 1e-12 for k in range(46 * 32)]) for nr in (1, 2)]
+    cases += [(nr, nr, [(k + 0.5) / 8 for k in range(480)]) for nr in (3, 4, 5)]
+    for nr, code, Ts in cases:
+        for T in Ts:
+            lib.chk_rys(code, ctypes.c_double(T), p(u), p(w))
+            F = O.boys(2 * nr - 1, T)
+            assert abs(np.sum(w[:nr]) - F[0]) <= 8e-16 * F[0], (nr, T)
+            for k in range(1, 2 * nr):
+                assert abs(np.sum(w[:nr] * u[:nr] ** k) - F[k]) <= 1e-15 * F[0] + 8e-15 * F[k], (nr, T, k)
