@@ -40,18 +40,26 @@ struct QClass {
 };
 
 // Split policy of the engine: slice over a (then over c) while a thread would
-// hold more than HFX_JK_SPLIT integrals (81; 54 and 27 measured slower).
-#ifndef HFX_JK_SPLIT
-#define HFX_JK_SPLIT 81
+// hold more integrals than the limit of its arithmetic type. Every slice
+// repeats the primitive set-up, the Rys rule and the recurrences, so fewer
+// slices pay as long as the accumulators fit in registers: FP64 108 (64
+// waters, FP64 build: 0.1352 s at 81, 0.1289 s at 108, 0.1373 s at 162;
+// 54 and 27 slower than 81), FP32 162 (registers of half the width).
+#ifndef HFX_JK_SPLIT_F64
+#define HFX_JK_SPLIT_F64 108
 #endif
-HFX_HD constexpr bool split_a(int la, int lb, int lc, int ld) {
-    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > HFX_JK_SPLIT;
+#ifndef HFX_JK_SPLIT_F32
+#define HFX_JK_SPLIT_F32 162
+#endif
+HFX_HD constexpr int jk_split_limit(bool fp32) { return fp32 ? HFX_JK_SPLIT_F32 : HFX_JK_SPLIT_F64; }
+HFX_HD constexpr bool split_a(int la, int lb, int lc, int ld, bool fp32 = false) {
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > jk_split_limit(fp32);
 }
-HFX_HD constexpr bool split_c(int la, int lb, int lc, int ld) {
-    return split_a(la, lb, lc, ld) && ncart(lb) * ncart(lc) * ncart(ld) > HFX_JK_SPLIT;
+HFX_HD constexpr bool split_c(int la, int lb, int lc, int ld, bool fp32 = false) {
+    return split_a(la, lb, lc, ld, fp32) && ncart(lb) * ncart(lc) * ncart(ld) > jk_split_limit(fp32);
 }
-HFX_HD constexpr int nsplit(int la, int lb, int lc, int ld) {
-    return (split_a(la, lb, lc, ld) ? ncart(la) : 1) * (split_c(la, lb, lc, ld) ? ncart(lc) : 1);
+HFX_HD constexpr int nsplit(int la, int lb, int lc, int ld, bool fp32 = false) {
+    return (split_a(la, lb, lc, ld, fp32) ? ncart(la) : 1) * (split_c(la, lb, lc, ld, fp32) ? ncart(lc) : 1);
 }
 
 // One Cartesian direction of one Rys root: 2-D integrals, bra and ket

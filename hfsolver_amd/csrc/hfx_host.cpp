@@ -219,6 +219,16 @@ int upload_temp(hfx_t* h, DeviceTemp& b, const std::vector<T>& v) {
     return 0;
 }
 
+// A non-finite element of an n x n input matrix (column-major) fails the call,
+// naming its position: the engine would otherwise spread it over G or g.
+int check_finite(hfx_t* h, const double* a, int n, const char* what) {
+    const size_t nn = (size_t)n * n;
+    for (size_t k = 0; k < nn; k++)
+        if (!std::isfinite(a[k]))
+            return fail(h, "%s: non-finite element %g at (%zu, %zu)", what, a[k], k % n, k / n);
+    return 0;
+}
+
 void free_geometry(hfx_t* h) {
     for (void* p : h->geom_allocs) (void)hipFree(p);
     h->geom_allocs.clear();
@@ -712,11 +722,17 @@ int build_lists(hfx_t* h) {
             const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
             rc.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
             rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
-            int nrow = 0, nab = 0;
-            hfx_row_shape(cb, ck, &nrow, &nab);
+            // the FP64 and FP32 launches slice differently: plan for the larger LDS of the two
+            int nrow64 = 0, nab64 = 0, nrow32 = 0, nab32 = 0;
+            hfx_row_shape(cb, ck, 0, &nrow64, &nab64);
+            hfx_row_shape(cb, ck, 1, &nrow32, &nab32);
+            auto row_lds = [&](int R, int dl) {
+                return std::max(hfx_row_lds(nrow64, nab64, rc.ncols, R, dl, false),
+                                hfx_row_lds(nrow32, nab32, rc.ncols, R, dl, true));
+            };
             int dlds = h->row_dlds;
-            if (dlds && hfx_row_lds(nrow, nab, rc.ncols, 1, 1, false) > h->lds_budget) dlds = 0;
-            const size_t need = hfx_row_lds(nrow, nab, rc.ncols, 1, dlds, false);
+            if (dlds && row_lds(1, 1) > h->lds_budget) dlds = 0;
+            const size_t need = row_lds(1, dlds);
             // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
             int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
             if (R == 0 && need <= h->lds_budget) R = 1;
@@ -1147,6 +1163,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
 
 int hfx_fock_2e(hfx_t* h, const double* P, double* G, double* J, double* K, hfx_stats* st) {
     if (!h || !P) return h ? fail(h, "hfx_fock_2e: null density") : 1;
+    if (check_finite(h, P, h->nbf, "hfx_fock_2e: density")) return 1;
     HIPCHK(h, hipSetDevice(h->device));
     const size_t nn = (size_t)h->nbf * h->nbf;
     if (ensure_work(h)) return 1;
@@ -1192,6 +1209,9 @@ int hfx_one_electron(hfx_t* h, double* S, double* T, double* V) {
 
 int hfx_grad_1e(hfx_t* h, const double* P, const double* W, double* g) {
     if (!h || !P || !W || !g) return h ? fail(h, "hfx_grad_1e: null argument") : 1;
+    if (check_finite(h, P, h->nbf, "hfx_grad_1e: density") ||
+        check_finite(h, W, h->nbf, "hfx_grad_1e: energy-weighted density"))
+        return 1;
     HIPCHK(h, hipSetDevice(h->device));
     if (ensure_work(h)) return 1;
     if (3 * (size_t)h->nat * sizeof(double) > HFX_ROW_LDS_MAX)
@@ -1220,6 +1240,7 @@ int hfx_grad_1e(hfx_t* h, const double* P, const double* W, double* g) {
 
 int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     if (!h || !P || !g) return h ? fail(h, "hfx_grad_2e: null argument") : 1;
+    if (check_finite(h, P, h->nbf, "hfx_grad_2e: density")) return 1;
     HIPCHK(h, hipSetDevice(h->device));
     const int n = h->nbf;
     const size_t nn = (size_t)n * n;
@@ -1252,7 +1273,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         const auto& c = h->rcombos[o.second / 2];
         const int prec = o.second % 2;
         int nrow = 0, nab = 0;
-        hfx_row_shape(c.cb, c.ck, &nrow, &nab);
+        hfx_row_shape(c.cb, c.ck, 2, &nrow, &nab);
         int dlds = c.dlds;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) dlds = 0;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) continue;

@@ -227,7 +227,9 @@ static constexpr int HFX_TILE_COLS = 64;
 
 hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, int ntiles, hipStream_t s);
 // LDS K rows (NAt + NB) and J_ab entries (NAt * NB) per bra row of a row-mode launch
-void hfx_row_shape(int cls_bra, int cls_ket, int* nrow, int* nab);
+// LDS rows per bra row (nrow) and J_ab entries (nab) of a row-mode launch:
+// kind 0 = J/K in FP64, 1 = J/K in FP32, 2 = gradient (their slicing differs)
+void hfx_row_shape(int cls_bra, int cls_ket, int kind, int* nrow, int* nab);
 hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, const RowArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad(int cls_bra, int cls_ket, bool fp32, const GradArgs& a, int nblocks, hipStream_t s);
 // LDS bytes of a row-mode gradient launch (density rows as in hfx_row_lds, then 3 nat doubles)

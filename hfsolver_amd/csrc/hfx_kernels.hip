@@ -71,7 +71,8 @@ __device__ HFX_INLINE double wave_sum(double v) {
 // the wavefront before a single atomic per element.
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
-    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    constexpr bool F32 = sizeof(R) == 4;
+    constexpr bool SPA = split_a(LA, LB, LC, LD, F32), SPC = split_c(LA, LB, LC, LD, F32);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     // the Boys table is read through the caches, as in the row-mode kernels
     const double* boys = hfx_dev::BOYS_TAB;
@@ -143,7 +144,8 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 // Row mode: one block = R bra shell pairs x a chunk of the ket table (see RowArgs).
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(HFX_ROW_THREADS) void jk_row_kernel(RowArgs a) {
-    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    constexpr bool F32 = sizeof(R) == 4;
+    constexpr bool SPA = split_a(LA, LB, LC, LD, F32), SPC = split_c(LA, LB, LC, LD, F32);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
     constexpr int nrow = NAt + NB;             // LDS K rows per bra row
@@ -751,11 +753,6 @@ __global__ void finish_kernel(const double* __restrict__ Jb, const double* __res
 }
 #endif
 
-template <int LA, int LB, int LC, int LD>
-constexpr int row_nab() {
-    return (split_a(LA, LB, LC, LD) ? 1 : ncart(LA)) + ncart(LB);
-}
-
 // table of a function template F<X, Y> over class combinations X >= Y
 #define HFX_TROW(F, X)                                                                                    \
     {F<X, 0>, X >= 1 ? F<X, (X >= 1 ? 1 : 0)> : nullptr, X >= 2 ? F<X, (X >= 2 ? 2 : 0)> : nullptr,        \
@@ -767,7 +764,7 @@ constexpr int row_nab() {
 #if HFX_PART == 1
 template <int X, int Y>
 hipError_t jk_xy(bool fp32, const JKArgs& a, int ntiles, hipStream_t s) {
-    dim3 grid((unsigned)ntiles, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    dim3 grid((unsigned)ntiles, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y), fp32));
     if (fp32)
         hipLaunchKernelGGL((jk_kernel<HFX_CLS(X), HFX_CLS(Y), float>), grid, dim3(HFX_TILE_ROWS * HFX_TILE_COLS), 0, s, a);
     else
@@ -783,9 +780,10 @@ const jk_fn JK_TABLE[6][6] = HFX_TABLE(jk_xy);
 typedef HFX_REAL_T RowReal;
 template <int X, int Y>
 hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
-    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
+    constexpr bool F32 = sizeof(RowReal) == 4;
+    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y), F32) ? 1 : ncart(pair_class_la(X));
     constexpr int NB = ncart(pair_class_lb(X));
-    const size_t lds = hfx_row_lds(NAt + NB, NAt * NB, a.ncols, a.R, a.dlds, sizeof(RowReal) == 4);
+    const size_t lds = hfx_row_lds(NAt + NB, NAt * NB, a.ncols, a.R, a.dlds, F32);
     if (lds > HFX_ROW_LDS_MAX || a.R < 1 || a.R > HFX_ROWS_MAX || a.kt.S < 1 || a.kt.S > HFX_SUB_MAX)
         return hipErrorInvalidValue;
     static bool attr_set = false;
@@ -795,7 +793,7 @@ hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y)));
+    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y), F32));
     hipLaunchKernelGGL((jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>), grid, dim3(HFX_ROW_THREADS), lds, s, a);
     return hipGetLastError();
 }
@@ -852,13 +850,17 @@ hipError_t block_xy(const BlockArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 template <int X, int Y>
-void row_shape_xy(int* nrow, int* nab) {
-    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
-    *nrow = NAt + ncart(pair_class_lb(X));
-    *nab = NAt * ncart(pair_class_lb(X));
+void row_shape_xy(int kind, int* nrow, int* nab) {
+    constexpr int NA = ncart(pair_class_la(X)), NB = ncart(pair_class_lb(X));
+    constexpr int NAt64 = split_a(HFX_CLS(X), HFX_CLS(Y), false) ? 1 : NA;
+    constexpr int NAt32 = split_a(HFX_CLS(X), HFX_CLS(Y), true) ? 1 : NA;
+    constexpr int NAtg = gsplit_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : NA;
+    const int NAt = kind == 0 ? NAt64 : kind == 1 ? NAt32 : NAtg;
+    *nrow = NAt + NB;
+    *nab = NAt * NB;
 }
 typedef hipError_t (*blk_fn)(const BlockArgs&, hipStream_t);
-typedef void (*shape_fn)(int*, int*);
+typedef void (*shape_fn)(int, int*, int*);
 const blk_fn BLK_TABLE[6][6] = HFX_TABLE(block_xy);
 const shape_fn SHAPE_TABLE[6][6] = HFX_TABLE(row_shape_xy);
 #endif
@@ -921,10 +923,10 @@ int hfx_nsplit(int cb, int ck) {
     return nsplit(pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck));
 }
 
-void hfx_row_shape(int cb, int ck, int* nrow, int* nab) {
+void hfx_row_shape(int cb, int ck, int kind, int* nrow, int* nab) {
     *nrow = *nab = 0;
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb) return;
-    SHAPE_TABLE[cb][ck](nrow, nab);
+    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || kind < 0 || kind > 2) return;
+    SHAPE_TABLE[cb][ck](kind, nrow, nab);
 }
 
 hipError_t hfx_launch_jk_rows(int cb, int ck, bool fp32, const RowArgs& a, int nblocks, hipStream_t s) {

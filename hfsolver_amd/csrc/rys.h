@@ -91,13 +91,21 @@ HFX_HD HFX_INLINE double hfx_rcp(double x) {
 #endif
 }
 
+// Two-root rule for T >= BOYS_TMAX (moments (2m-1)!! F_0 / (2T)^m): nodes
+// v / (2T) with v = 3 -+ sqrt(6), weights F_0 (sqrt(6) +- 2) / (2 sqrt(6)).
+#define HFX_RYS2_V1 0.55051025721682190180   // 3 - sqrt(6)
+#define HFX_RYS2_V2 5.44948974278317809820   // 3 + sqrt(6)
+#define HFX_RYS2_W1 0.90824829046386301637   // (sqrt(6) + 2) / (2 sqrt(6))
+#define HFX_RYS2_W2 0.09175170953613698363   // (sqrt(6) - 2) / (2 sqrt(6))
+
 // One- and two-root rules in closed form from the Boys moments F_0..F_{2n-1}.
 // F_m(T) for T < BOYS_TMAX: Taylor series about the nearest grid point T0,
 //   F_m(T) = sum_{k < BOYS_K} F_{m+k}(T0) (T0 - T)^k / k!;
 // beyond: F_0 = (1/2) sqrt(pi/T) (erfc(sqrt T) < 1e-16 F_0 there) and the
 // upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), in which e^{-T}
 // (below 1.1e-20 for T >= 46) is under 1e-16 of (2m+1) F_m for m <= 2 and is
-// left out, so that no exponential is evaluated.
+// left out, so that no exponential is evaluated; the rules then have the
+// closed forms above (HFX_RYS2_*).
 // n = 1: u = F_1/F_0, w = F_0. n = 2: the orthogonal quadratic of the moment
 // sequence, D u^2 + A u + B with D = F_0 F_2 - F_1^2, A = F_1 F_2 - F_0 F_3,
 // B = F_1 F_3 - F_2^2; with E = (-A + sqrt(A^2 - 4 D B))/2 the roots are
@@ -130,14 +138,20 @@ HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const dou
     } else {
         const double rt = hfx_rsqrt(T);
         const double it2 = 0.5 * rt * rt;
-        F[0] = 0.5 * HFX_SQRT_PI * rt;
+        const double f0 = 0.5 * HFX_SQRT_PI * rt;
         if constexpr (NR == 1) {
             u[0] = it2;
-            w[0] = F[0];
+            w[0] = f0;
         } else {
-#pragma unroll
-            for (int m = 0; m + 1 < NM; m++) F[m + 1] = (2 * m + 1) * F[m] * it2;
+            // moments F_m = (2m-1)!! F_0 / (2T)^m: the orthogonal quadratic is
+            // v^2 - 6v + 3 in v = 2T u, so u = (3 -+ sqrt 6)/(2T) and
+            // w = F_0 (sqrt 6 +- 2)/(2 sqrt 6)
+            u[0] = HFX_RYS2_V1 * it2;
+            u[1] = HFX_RYS2_V2 * it2;
+            w[0] = HFX_RYS2_W1 * f0;
+            w[1] = HFX_RYS2_W2 * f0;
         }
+        return;
     }
     if constexpr (NR == 2) {
         const double m0 = F[0], m1 = F[1], m2 = F[2], m3 = F[3];
@@ -157,7 +171,7 @@ HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const dou
 
 // The same rules in single precision for the FP32 kernels, from BOYSF_TAB
 // (BOYSF_K Taylor terms, relative error below float rounding); beyond
-// BOYS_TMAX the two-root rule keeps e^{-T} (float exp).
+// BOYS_TMAX the same asymptotic closed forms.
 template <int NR>
 HFX_HD HFX_INLINE void rys_roots_small_f(float T, float* u, float* w, const float* boysf) {
     static_assert(NR == 1 || NR == 2, "closed form for one or two roots only");
@@ -182,21 +196,21 @@ HFX_HD HFX_INLINE void rys_roots_small_f(float T, float* u, float* w, const floa
             w[0] = F[0];
         }
     } else {
+        // the asymptotic rules of rys_roots_small (no moments: their products
+        // would leave the single-precision range for large T)
         const float rt = hfx_rsqrt(T);
         const float it2 = 0.5f * rt * rt;
-        F[0] = 0.5f * (float)HFX_SQRT_PI * rt;
+        const float f0 = 0.5f * (float)HFX_SQRT_PI * rt;
         if constexpr (NR == 1) {
             u[0] = it2;
-            w[0] = F[0];
+            w[0] = f0;
         } else {
-#if defined(__HIP_DEVICE_COMPILE__)
-            const float et = __expf(-T);
-#else
-            const float et = std::exp(-T);
-#endif
-#pragma unroll
-            for (int m = 0; m + 1 < NM; m++) F[m + 1] = ((2 * m + 1) * F[m] - et) * it2;
+            u[0] = (float)HFX_RYS2_V1 * it2;
+            u[1] = (float)HFX_RYS2_V2 * it2;
+            w[0] = (float)HFX_RYS2_W1 * f0;
+            w[1] = (float)HFX_RYS2_W2 * f0;
         }
+        return;
     }
     if constexpr (NR == 2) {
         const float m0 = F[0], m1 = F[1], m2 = F[2], m3 = F[3];

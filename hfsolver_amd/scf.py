@@ -52,6 +52,13 @@ def nuclear_gradient(atno, xyz):
     return g
 
 
+def _check_energy(it, E0):
+    """A non-finite electronic energy (NaN or Inf in G or in the density) stops
+    the SCF; the reference would carry on into LAPACK with it."""
+    if not math.isfinite(E0):
+        raise FloatingPointError("SCF iteration %d: non-finite electronic energy %r" % (it, E0))
+
+
 def _diis_extrapolate(Fs, errs):
     m = len(Fs)
     B = -np.ones((m + 1, m + 1))
@@ -178,6 +185,7 @@ def _device_step(it, P, G, Ht, St, Li, nocc, Escf, Fs, errs, diis, diis_size, ca
     import torch
     F = Ht + G
     E0 = 0.5 * float(torch.sum(P * (Ht + F)))
+    _check_energy(it, E0)
     Escf.append(E0)
     if callback is not None:
         callback(it, E0, P, G)
@@ -265,6 +273,9 @@ def doscf(H, S, nocc, fock_2e, Nscf=40, alpha=1.0, tolE=1e-10, tolP=1e-8, diis=F
         G = fock_2e(P)
         F = H + G
         E0 = 0.5 * float(np.sum(P * (H + F)))
+        _check_energy(it, E0)
+        if not np.all(np.isfinite(G)):
+            raise FloatingPointError("SCF iteration %d: the two-electron matrix has non-finite elements" % it)
         Escf.append(E0)
         if callback is not None:
             callback(it, E0, P, G)

@@ -77,7 +77,9 @@ int hfx_create(int nbf, const int *nprim, const int *istart, const double *x, co
  * Replaces G = formG(P, TT) (src/scf.f90:26-37, :109):
  *   J_ij = sum_kl P_kl (ij|kl),  K_ij = sum_kl P_kl (il|kj),  G = J - K/2.
  * P: n x n input (symmetrised as (P + P^T)/2). G, J, K: n x n outputs; any of
- * them may be NULL. stats may be NULL.
+ * them may be NULL. stats may be NULL. A non-finite element of P (NaN, Inf)
+ * fails the call (status 1, its position in hfx_last_error); the same holds
+ * for the P and W of hfx_grad_2e and hfx_grad_1e.
  */
 int hfx_fock_2e(hfx_t *h, const double *P, double *G, double *J, double *K, hfx_stats *stats);
 
@@ -86,7 +88,9 @@ int hfx_fock_2e(hfx_t *h, const double *P, double *G, double *J, double *K, hfx_
  * HBM): no host transfers. Used by benchmarks and by callers that keep the
  * density on the GPU. The engine works on streams of its own: the writes that
  * produced dP must be complete when it is called (synchronise the producing
- * stream first). Synchronous: dG, dJ, dK are complete on return.
+ * stream first). Synchronous: dG, dJ, dK are complete on return. dP is not
+ * scanned: a non-finite element gives non-finite elements of dG (the SCF
+ * drivers stop on a non-finite energy).
  */
 int hfx_fock_2e_device(hfx_t *h, const double *dP, double *dG, double *dJ, double *dK, hfx_stats *stats);
 

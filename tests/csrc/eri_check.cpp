@@ -56,9 +56,10 @@ void make_pair(int la, int lb, const double* A, const double* B, int na, const d
 
 template <int LA, int LB, int LC, int LD, typename R>
 void run(const ShellPair& bra, const ShellPair& ket, const PrimPair* prims, double* out) {
-    constexpr bool SPA = split_a(LA, LB, LC, LD), SPC = split_c(LA, LB, LC, LD);
+    constexpr bool F32 = sizeof(R) == 4;   // the slicing of the FP32 kernels
+    constexpr bool SPA = split_a(LA, LB, LC, LD, F32), SPC = split_c(LA, LB, LC, LD, F32);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
-    for (int split = 0; split < nsplit(LA, LB, LC, LD); split++) {
+    for (int split = 0; split < nsplit(LA, LB, LC, LD, F32); split++) {
         const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
         const int sc = SPC ? split % Q::NC : 0;
         R eri[Q::NOUT];

@@ -11,8 +11,20 @@ SRC = os.path.join(HERE, "csrc", "eri_check.cpp")
 LIB = os.path.join(HERE, "csrc", "liberi_check.so")
 
 
+SAN_SRC = os.path.join(HERE, "csrc", "san_check.cpp")
+SAN_EXE = os.path.join(HERE, "csrc", "san_check")
+
+
 def build():
     subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", SRC, "-o", LIB], check=True)
+    build_sanitized()
+
+
+def build_sanitized():
+    """The harness and its driver (san_check.cpp) as one program built with
+    AddressSanitizer and UndefinedBehaviorSanitizer (host code only)."""
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer", SRC, SAN_SRC, "-o", SAN_EXE], check=True)
 
 
 def load():

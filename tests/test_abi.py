@@ -81,3 +81,20 @@ def test_cython_module_loads_and_fails_loudly_without_gpu():
         return
     eng.close()
     pytest.skip("a HIP device is present; covered by the gpu tests")
+
+
+def test_empty_basis_is_rejected():
+    """nbf = 0 (or a null array) fails hfx_create with a message, before any device work."""
+    import ctypes
+    L = E.lib()
+    o = E.HfxOpts()
+    L.hfx_default_opts(ctypes.byref(o))
+    h = ctypes.c_void_p()
+    z = np.zeros(1, dtype=np.int32)
+    d = np.zeros(1)
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    rc = L.hfx_create(0, p(z), p(z), p(d), p(d), p(d), p(z), p(z), p(z), 1, p(d), p(d), 1, p(z), p(d),
+                      ctypes.byref(o), ctypes.byref(h))
+    assert rc != 0
+    assert b"invalid arguments" in L.hfx_last_error(h)
+    L.hfx_destroy(h)

@@ -261,3 +261,24 @@ def test_device_entry_waits_for_the_producing_stream():
     G = dG.cpu().numpy().reshape(b.nbf, b.nbf, order="F")
     assert np.abs(G - G_ref).max() < 1e-12
     eng.close()
+
+
+def test_non_finite_density_is_rejected():
+    """hfx_fock_2e / hfx_grad_2e reject a density with NaN or Inf, naming the
+    element, and the engine stays usable."""
+    from hfsolver_amd.engine import HfxError
+    atno, xyz = water_cluster(2, seed=2)
+    eng = Engine(Basis(atno, xyz))
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((eng.n, 6))
+    P = X @ X.T / 6
+    G0 = eng.fock_2e(P)[0]
+    for bad in (np.nan, np.inf):
+        Pb = P.copy()
+        Pb[3, 5] = bad
+        with pytest.raises(HfxError, match=r"non-finite element .* at \(3, 5\)"):
+            eng.fock_2e(Pb)
+        with pytest.raises(HfxError, match="non-finite"):
+            eng.grad_2e(Pb)
+    assert np.abs(eng.fock_2e(P)[0] - G0).max() < 1e-12
+    eng.close()

@@ -43,3 +43,37 @@ def test_nuclear_gradient_finite_difference():
             xm[a, d] -= h
             fd = (nuclear_energy(atno, xp) - nuclear_energy(atno, xm)) / (2 * h)
             assert abs(fd - g[a, d]) < 1e-8
+
+
+def test_scf_that_does_not_converge_raises(golden):
+    """doscf stops after Nscf iterations without convergence with an error, as
+    the reference's stop_error does (src/scf.f90:135)."""
+    import pytest
+    g = golden("h2o_config0")
+    B, S, H = _setup(g)
+    with pytest.raises(RuntimeError, match="did not converge in 3 iterations"):
+        doscf(H, S, 5, lambda P: O.formG(B, P), 3, 1.0, 1e-10, 1e-8)
+
+
+def test_non_finite_two_electron_matrix_stops_the_scf(golden):
+    import pytest
+    g = golden("h2o_config0")
+    B, S, H = _setup(g)
+    calls = []
+
+    def bad(P):
+        G = O.formG(B, P)
+        calls.append(1)
+        if len(calls) == 3:
+            G[4, 7] = np.nan
+        return G
+    with pytest.raises(FloatingPointError, match="iteration 3"):
+        doscf(H, S, 5, bad, 40, 1.0, 1e-10, 1e-8)
+
+
+def test_unknown_element_is_rejected():
+    """An atomic number with no 6-31G** entry is an error (src/basis.f90:213-215)."""
+    import pytest
+    from hfsolver_amd.basis import Basis
+    with pytest.raises(ValueError, match="Z=99"):
+        Basis([99, 1], [[0.0, 0.0, 0.0], [1.4, 0.0, 0.0]])
