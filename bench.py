@@ -159,6 +159,33 @@ def restatement_full_build(nwater=16, seed=16):
             "quartets": nq}, (atno, xyz, P)
 
 
+def profile_builds(args):
+    """One FP64 and one mixed-precision Fock build of the workload, nothing else
+    timed: the run the rocprofv3 counter passes of the committed profiles use."""
+    import scipy.linalg
+    import torch
+
+    from hfsolver_amd.basis import Basis
+    from hfsolver_amd.clusters import water_cluster
+    from hfsolver_amd.engine import Engine
+    atno, xyz = water_cluster(args.nwater, seed=args.seed)
+    b = Basis(atno, xyz)
+    e = Engine(b)
+    S, T, V = e.one_electron()
+    nocc = int(sum(atno)) // 2
+    lam, C = scipy.linalg.eigh(T + V, S, lower=True, driver="gvd")
+    dP = torch.from_numpy(np.ascontiguousarray(2.0 * C[:, :nocc] @ C[:, :nocc].T)).to("cuda:0")
+    dG = torch.zeros_like(dP)
+    for name, thr in (("fp64", 0.0), ("mixed", args.fp32_threshold)):
+        eng = Engine(b, fp32_threshold=thr, verbose=2 if args.verbose else 0)   # 2: per-class model FLOPs
+        torch.cuda.synchronize()
+        st = eng.fock_2e_device(dP, dG)
+        log("  profile build %s: %.4f s (J/K kernels %.4f s)" % (name, st.seconds, st.jk_seconds))
+        eng.close()
+    e.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +213,10 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known bytes) to scale HBM counters")
+    ap.add_argument("--profile-builds", action="store_true",
+                    help="only run one FP64 and one mixed build (device buffers, core-Hamiltonian density) for "
+                         "profiler counters; each build starts with one prep_kernel dispatch (with --verbose: "
+                         "the modelled FLOPs of every class launch)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -197,6 +228,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 3}))
         return 0
+    if args.profile_builds:
+        return profile_builds(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured mode).", file=sys.stderr)

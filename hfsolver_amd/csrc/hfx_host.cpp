@@ -76,6 +76,7 @@ struct hfx_engine {
     std::vector<int> bfL[HFX_LMAX + 1];   // functions of the shells of each L, in shell order
     int* d_bfL[HFX_LMAX + 1] = {nullptr};
     size_t lds_budget = 120 * 1024;   // row-mode LDS per block (the heaviest classes run one block per CU)
+    size_t dlds_budget = 120 * 1024;  // largest row-mode LDS per block with the density rows staged
     // pairs
     std::vector<ShellPair> pairs;
     std::vector<PrimPair> prims;
@@ -731,7 +732,7 @@ int build_lists(hfx_t* h) {
                                 hfx_row_lds(nrow32, nab32, rc.ncols, R, dl, true));
             };
             int dlds = h->row_dlds;
-            if (dlds && row_lds(1, 1) > h->lds_budget) dlds = 0;
+            if (dlds && row_lds(1, 1) > std::min(h->lds_budget, h->dlds_budget)) dlds = 0;
             const size_t need = row_lds(1, dlds);
             // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
             int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
@@ -974,6 +975,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (const char* e = std::getenv("HFX_ROW_RMAX")) h->row_rmax = std::max(1, std::min(HFX_ROWS_MAX, std::atoi(e)));
     if (const char* e = std::getenv("HFX_ROW_PAIR_KB")) h->row_pair_bytes = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
+    if (const char* e = std::getenv("HFX_ROW_DLDS_KB")) h->dlds_budget = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_BLOCKS")) h->row_blocks = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("HFX_STREAMS")) h->nstreams = std::max(1, std::min(HFX_AUX_STREAMS + 1, std::atoi(e)));
     h->nat = nat;
