@@ -217,16 +217,27 @@ struct BlockArgs {
 };
 
 static constexpr int HFX_ROWS_MAX = 4;   // bra rows per row-mode block
-#ifndef HFX_ROW_THREADS
-#define HFX_ROW_THREADS 256                // threads of a J/K row-mode block
-#endif
+// Threads of a J/K row-mode block of class (X|Y) (pair classes) in FP32 or FP64:
+// 256, or 512 / 1024 for the classes whose K and density rows hold a CU to one
+// or two blocks while their registers leave room for two or four wavefronts per
+// SIMD (chosen per class from launch times at 64 waters 6-31G**, n = 1600).
+HFX_HD constexpr int hfx_row_threads(int X, int Y, bool fp32) {
+    if (fp32) {
+        if (X == 3 && Y == 1) return 1024;                            // (20|10)
+        if ((X == 1 && Y == 1) || (X == 2 && Y == 1) || (X == 4 && Y == 1) || (X == 4 && Y == 3) ||
+            (X == 5 && Y == 4))
+            return 512;                                               // (10|10) (11|10) (21|10) (21|20) (22|21)
+        return 256;
+    }
+    if ((X == 2 && Y == 1) || (X == 3 && Y == 3) || (X == 5 && Y == 3)) return 512;   // (11|10) (20|20) (22|20)
+    return 256;
+}
 static constexpr int HFX_AUX_STREAMS = 3; // extra streams of the class launches (at most)
 static constexpr size_t HFX_ROW_LDS_MAX = 128 * 1024;   // dynamic LDS of a row-mode block
 static constexpr int HFX_TILE_ROWS = 4;   // one bra row per wavefront
 static constexpr int HFX_TILE_COLS = 64;
 
 hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, int ntiles, hipStream_t s);
-// LDS K rows (NAt + NB) and J_ab entries (NAt * NB) per bra row of a row-mode launch
 // LDS rows per bra row (nrow) and J_ab entries (nab) of a row-mode launch:
 // kind 0 = J/K in FP64, 1 = J/K in FP32, 2 = gradient (their slicing differs)
 void hfx_row_shape(int cls_bra, int cls_ket, int kind, int* nrow, int* nab);

@@ -143,7 +143,8 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 #endif
 // Row mode: one block = R bra shell pairs x a chunk of the ket table (see RowArgs).
 template <int LA, int LB, int LC, int LD, typename R>
-__global__ __launch_bounds__(HFX_ROW_THREADS) void jk_row_kernel(RowArgs a) {
+__global__ __launch_bounds__(hfx_row_threads(pair_class(LA, LB), pair_class(LC, LD), sizeof(R) == 4))
+void jk_row_kernel(RowArgs a) {
     constexpr bool F32 = sizeof(R) == 4;
     constexpr bool SPA = split_a(LA, LB, LC, LD, F32), SPC = split_c(LA, LB, LC, LD, F32);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
@@ -794,7 +795,8 @@ hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
         attr_set = true;
     }
     dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y), F32));
-    hipLaunchKernelGGL((jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>), grid, dim3(HFX_ROW_THREADS), lds, s, a);
+    hipLaunchKernelGGL((jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>), grid, dim3(hfx_row_threads(X, Y, F32)), lds,
+                       s, a);
     return hipGetLastError();
 }
 typedef hipError_t (*rows_fn)(const RowArgs&, int, hipStream_t);
