@@ -73,7 +73,7 @@ HFX_HD HFX_INLINE void rys_dir(R c00, R d00, R b10, R b01, R b00, R ab, R cd, R 
     constexpr int LAB = Q::LAB, LCD = Q::LCD;
     R G[LAB + 1][LCD + 1];
     G[0][0] = g00;
-    if (LAB > 0) G[1][0] = c00 * g00;
+    if constexpr (LAB > 0) G[1][0] = c00 * g00;
 #pragma unroll
     for (int n = 1; n < LAB; n++) G[n + 1][0] = c00 * G[n][0] + R(n) * b10 * G[n - 1][0];
 #pragma unroll
