@@ -19,21 +19,32 @@
 #include "eri_rys.h"
 
 // Split policy of the gradient kernels: slice over a (then over c) while a
-// thread would hold more than HFX_GRAD_SPLIT density factors gam. The
-// derivative tables need more registers than the energy's (limit 81); a lower
-// limit removes most register spills but repeats the primitive set-up in every
-// slice -- 54 was the fastest of 81 / 54 / 36 on 64 waters.
+// thread would hold more than grad_split_limit density factors gam. The
+// derivative tables need more registers than the energy's; a lower limit
+// removes register spills but repeats the primitive set-up in every slice.
+// 54 was the fastest of 81 / 54 / 36 on 64 waters over all classes; per class
+// (launches serialised, 64 waters) FP64 (22|11) runs 42 % faster at 27 (its
+// spills drop from 1212 to 116 bytes), FP32 (11|11) and (22|21) 27-28 %
+// faster at 108 (no slice over a, resp. c); every other class is slower at
+// 27 or at 108.
 #ifndef HFX_GRAD_SPLIT
 #define HFX_GRAD_SPLIT 54
 #endif
-HFX_HD constexpr bool gsplit_a(int la, int lb, int lc, int ld) {
-    return ncart(la) > 1 && ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > HFX_GRAD_SPLIT;
+HFX_HD constexpr int grad_split_limit(int la, int lb, int lc, int ld, bool fp32) {
+    return fp32 ? (((la == 1 && lb == 1 && lc == 1 && ld == 1) || (la == 2 && lb == 2 && lc == 2 && ld == 1))
+                       ? 2 * HFX_GRAD_SPLIT
+                       : HFX_GRAD_SPLIT)
+                : ((la == 2 && lb == 2 && lc == 1 && ld == 1) ? HFX_GRAD_SPLIT / 2 : HFX_GRAD_SPLIT);
 }
-HFX_HD constexpr bool gsplit_c(int la, int lb, int lc, int ld) {
-    return gsplit_a(la, lb, lc, ld) && ncart(lc) > 1 && ncart(lb) * ncart(lc) * ncart(ld) > HFX_GRAD_SPLIT;
+HFX_HD constexpr bool gsplit_a(int la, int lb, int lc, int ld, bool fp32) {
+    return ncart(la) > 1 && ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > grad_split_limit(la, lb, lc, ld, fp32);
 }
-HFX_HD constexpr int gnsplit(int la, int lb, int lc, int ld) {
-    return (gsplit_a(la, lb, lc, ld) ? ncart(la) : 1) * (gsplit_c(la, lb, lc, ld) ? ncart(lc) : 1);
+HFX_HD constexpr bool gsplit_c(int la, int lb, int lc, int ld, bool fp32) {
+    return gsplit_a(la, lb, lc, ld, fp32) && ncart(lc) > 1 &&
+           ncart(lb) * ncart(lc) * ncart(ld) > grad_split_limit(la, lb, lc, ld, fp32);
+}
+HFX_HD constexpr int gnsplit(int la, int lb, int lc, int ld, bool fp32) {
+    return (gsplit_a(la, lb, lc, ld, fp32) ? ncart(la) : 1) * (gsplit_c(la, lb, lc, ld, fp32) ? ncart(lc) : 1);
 }
 
 // Extents of the tables of the pass for centre CEN (0: A, 1: B, 2: C, 3: A and

@@ -1275,7 +1275,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         const auto& c = h->rcombos[o.second / 2];
         const int prec = o.second % 2;
         int nrow = 0, nab = 0;
-        hfx_row_shape(c.cb, c.ck, 2, &nrow, &nab);
+        hfx_row_shape(c.cb, c.ck, prec ? 3 : 2, &nrow, &nab);   // the launch's slicing (FP64 or FP32)
         int dlds = c.dlds;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) dlds = 0;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) continue;

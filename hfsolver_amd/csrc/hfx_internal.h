@@ -232,6 +232,12 @@ HFX_HD constexpr int hfx_row_threads(int X, int Y, bool fp32) {
     if ((X == 2 && Y == 1) || (X == 3 && Y == 3) || (X == 5 && Y == 3)) return 512;   // (11|10) (20|20) (22|20)
     return 256;
 }
+// Threads of a gradient row-mode block of class (X|Y) in FP32 or FP64: 256,
+// or 512 for the FP32 classes (21|11), (21|10), (20|11) (10-24 % faster per
+// class at 64 waters; every other class is slower with 512 or 1024).
+HFX_HD constexpr int hfx_grad_threads(int X, int Y, bool fp32) {
+    return fp32 && ((X == 4 && Y == 2) || (X == 4 && Y == 1) || (X == 3 && Y == 2)) ? 512 : 256;
+}
 static constexpr int HFX_AUX_STREAMS = 3; // extra streams of the class launches (at most)
 static constexpr size_t HFX_ROW_LDS_MAX = 128 * 1024;   // dynamic LDS of a row-mode block
 static constexpr int HFX_TILE_ROWS = 4;   // one bra row per wavefront
@@ -239,7 +245,8 @@ static constexpr int HFX_TILE_COLS = 64;
 
 hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, int ntiles, hipStream_t s);
 // LDS rows per bra row (nrow) and J_ab entries (nab) of a row-mode launch:
-// kind 0 = J/K in FP64, 1 = J/K in FP32, 2 = gradient (their slicing differs)
+// kind 0 = J/K in FP64, 1 = J/K in FP32, 2 = gradient in FP64, 3 = gradient
+// in FP32 (their slicing differs)
 void hfx_row_shape(int cls_bra, int cls_ket, int kind, int* nrow, int* nab);
 hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, const RowArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad(int cls_bra, int cls_ket, bool fp32, const GradArgs& a, int nblocks, hipStream_t s);

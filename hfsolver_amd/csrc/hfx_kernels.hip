@@ -401,8 +401,10 @@ void jk_row_kernel(RowArgs a) {
 // D_ad D_bc); A and B components reduced over the wavefront per row, C and D
 // (= -(A + B + C), translational invariance) by LDS atomics on their atoms.
 template <int LA, int LB, int LC, int LD, typename R>
-__global__ __launch_bounds__(256) void grad_row_kernel(GradRowArgs a) {
-    constexpr bool SPA = gsplit_a(LA, LB, LC, LD), SPC = gsplit_c(LA, LB, LC, LD);
+__global__ __launch_bounds__(hfx_grad_threads(pair_class(LA, LB), pair_class(LC, LD), sizeof(R) == 4))
+void grad_row_kernel(GradRowArgs a) {
+    constexpr bool G32 = sizeof(R) == 4;
+    constexpr bool SPA = gsplit_a(LA, LB, LC, LD, G32), SPC = gsplit_c(LA, LB, LC, LD, G32);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
     constexpr int nrow = NAt + NB;
@@ -562,7 +564,8 @@ __global__ __launch_bounds__(256) void grad_row_kernel(GradRowArgs a) {
 // Two-electron gradient: one block = one bra shell pair x a chunk of its kets.
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
-    constexpr bool SPA = gsplit_a(LA, LB, LC, LD), SPC = gsplit_c(LA, LB, LC, LD);
+    constexpr bool G32 = sizeof(R) == 4;
+    constexpr bool SPA = gsplit_a(LA, LB, LC, LD, G32), SPC = gsplit_c(LA, LB, LC, LD, G32);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     extern __shared__ double smem[];
     double* gl = smem;  // 3 * nat
@@ -816,7 +819,7 @@ hipError_t grad_xy(const GradArgs& a, int nblocks, hipStream_t s) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    dim3 grid((unsigned)nblocks, (unsigned)gnsplit(HFX_CLS(X), HFX_CLS(Y)));
+    dim3 grid((unsigned)nblocks, (unsigned)gnsplit(HFX_CLS(X), HFX_CLS(Y), sizeof(GradReal) == 4));
     hipLaunchKernelGGL((grad_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }
@@ -825,7 +828,7 @@ const grad_fn GRAD_TABLE[6][6] = HFX_TABLE(grad_xy);
 
 template <int X, int Y>
 hipError_t grad_rows_xy(const GradRowArgs& a, int nblocks, hipStream_t s) {
-    constexpr int NAt = gsplit_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : ncart(pair_class_la(X));
+    constexpr int NAt = gsplit_a(HFX_CLS(X), HFX_CLS(Y), sizeof(GradReal) == 4) ? 1 : ncart(pair_class_la(X));
     constexpr int NB = ncart(pair_class_lb(X));
     const size_t lds = hfx_grad_row_lds(NAt + NB, a.ncols, a.R, a.dlds, sizeof(GradReal) == 4, a.nat);
     if (lds > HFX_ROW_LDS_MAX || a.R < 1 || a.R > HFX_ROWS_MAX || a.kt.S < 1 || a.kt.S > HFX_SUB_MAX)
@@ -837,8 +840,9 @@ hipError_t grad_rows_xy(const GradRowArgs& a, int nblocks, hipStream_t s) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    dim3 grid((unsigned)nblocks, (unsigned)gnsplit(HFX_CLS(X), HFX_CLS(Y)));
-    hipLaunchKernelGGL((grad_row_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>), grid, dim3(256), lds, s, a);
+    dim3 grid((unsigned)nblocks, (unsigned)gnsplit(HFX_CLS(X), HFX_CLS(Y), sizeof(GradReal) == 4));
+    hipLaunchKernelGGL((grad_row_kernel<HFX_CLS(X), HFX_CLS(Y), GradReal>), grid,
+                       dim3(hfx_grad_threads(X, Y, sizeof(GradReal) == 4)), lds, s, a);
     return hipGetLastError();
 }
 typedef hipError_t (*grad_rows_fn)(const GradRowArgs&, int, hipStream_t);
@@ -856,8 +860,9 @@ void row_shape_xy(int kind, int* nrow, int* nab) {
     constexpr int NA = ncart(pair_class_la(X)), NB = ncart(pair_class_lb(X));
     constexpr int NAt64 = split_a(HFX_CLS(X), HFX_CLS(Y), false) ? 1 : NA;
     constexpr int NAt32 = split_a(HFX_CLS(X), HFX_CLS(Y), true) ? 1 : NA;
-    constexpr int NAtg = gsplit_a(HFX_CLS(X), HFX_CLS(Y)) ? 1 : NA;
-    const int NAt = kind == 0 ? NAt64 : kind == 1 ? NAt32 : NAtg;
+    constexpr int NAtg64 = gsplit_a(HFX_CLS(X), HFX_CLS(Y), false) ? 1 : NA;
+    constexpr int NAtg32 = gsplit_a(HFX_CLS(X), HFX_CLS(Y), true) ? 1 : NA;
+    const int NAt = kind == 0 ? NAt64 : kind == 1 ? NAt32 : kind == 2 ? NAtg64 : NAtg32;
     *nrow = NAt + NB;
     *nab = NAt * NB;
 }
@@ -927,7 +932,7 @@ int hfx_nsplit(int cb, int ck) {
 
 void hfx_row_shape(int cb, int ck, int kind, int* nrow, int* nab) {
     *nrow = *nab = 0;
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || kind < 0 || kind > 2) return;
+    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || kind < 0 || kind > 3) return;
     SHAPE_TABLE[cb][ck](kind, nrow, nab);
 }
 

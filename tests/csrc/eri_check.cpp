@@ -78,10 +78,11 @@ void run(const ShellPair& bra, const ShellPair& ket, const PrimPair* prims, doub
 template <int LA, int LB, int LC, int LD, typename R>
 void run_grad(const ShellPair& bra, const ShellPair& ket, const PrimPair* prims, const PrimExp* pexp,
               const double* gam_full, double* g) {
-    constexpr bool SPA = gsplit_a(LA, LB, LC, LD), SPC = gsplit_c(LA, LB, LC, LD);
+    constexpr bool G32 = sizeof(R) == 4;
+    constexpr bool SPA = gsplit_a(LA, LB, LC, LD, G32), SPC = gsplit_c(LA, LB, LC, LD, G32);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int split = 0; split < gnsplit(LA, LB, LC, LD); split++) {
+    for (int split = 0; split < gnsplit(LA, LB, LC, LD, G32); split++) {
         const int sa = SPA ? (SPC ? split / Q::NC : split) : 0;
         const int sc = SPC ? split % Q::NC : 0;
         R gam[Q::NOUT];
