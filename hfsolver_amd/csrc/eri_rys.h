@@ -169,22 +169,30 @@ struct KetPrimsAoS {
     }
 };
 
-// Ket primitives of a ket list stored field-major, so that consecutive kets
-// (consecutive lanes) read consecutive addresses: field f of primitive k of
-// ket j at base[(k * KETPRIM_FIELDS + f) * stride + j], base already offset by j.
+// Ket primitives of ket j of a sublist stored in the chunked planes of
+// KetChunk (tab: the sublist's table, pitch: its plane length).
 template <typename T>
 struct KetPrimsSoA {
-    const T* base;
+    const T* tab;
+    int j;
     int np;
-    int stride;
+    int pitch;
+    using C = typename KetChunkOf<T>::type;
+    static constexpr int PER = KetChunk<T>::PER, NCH = KetChunk<T>::NCH;
     HFX_HD HFX_INLINE int count() const { return np; }
     HFX_HD HFX_INLINE KetPrimT<T> get(int k) const {
-        const T* b = base + (size_t)k * KETPRIM_FIELDS * stride;
-        return KetPrimT<T>{b[0], b[stride], b[2 * stride], b[3 * stride], b[4 * stride],
-                           b[5 * stride], b[6 * stride], b[7 * stride], b[8 * stride]};
+        const T* pk = tab + (size_t)k * KETPRIM_FIELDS * pitch;
+        T f[8];
+#pragma unroll
+        for (int h = 0; h < NCH; h++) {
+            const C c = reinterpret_cast<const C*>(pk + (size_t)h * pitch * PER)[j];
+#pragma unroll
+            for (int e = 0; e < PER; e++) f[h * PER + e] = c.v[e];
+        }
+        return KetPrimT<T>{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], pk[(size_t)8 * pitch + j]};
     }
     // exponent of the C primitive of primitive pair k
-    HFX_HD HFX_INLINE T cexp(int k) const { return base[((size_t)k * KETPRIM_FIELDS + 9) * stride]; }
+    HFX_HD HFX_INLINE T cexp(int k) const { return tab[((size_t)k * KETPRIM_FIELDS + 9) * pitch + j]; }
 };
 
 // Ket primitives as PrimPair records plus their exponents (derivative integrals).
@@ -193,10 +201,19 @@ struct KetPrimsAoSG : KetPrimsAoS {
     HFX_HD HFX_INLINE double cexp(int k) const { return pe[k].aa; }
 };
 
-// Rys rule of the FP32 kernels: 0 = the FP64 rule (Boys function in FP64),
-// 1 = the single-precision rules of rys.h.
+// Rys rule of the FP32 kernels: 0 = the FP64 rules for every root count,
+// 1 = the single-precision rules of rys.h for every root count, 2 = one and
+// two roots from the Boys moments in FP64, three to five roots from the
+// single-precision fits (relative error < 1.1e-8, below the rounding of the
+// nodes and weights to float).
 #ifndef HFX_FP32_RULES
-#define HFX_FP32_RULES 0
+#define HFX_FP32_RULES 2
+#endif
+// Taylor terms of the FP64 Boys moments in the FP32 kernels: 5 terms truncate
+// at (1/64)^5/5! < 8e-12 relative, far below the float rounding of the nodes
+// and weights they give (the FP64 kernels take all BOYS_K = 7).
+#ifndef HFX_BOYS_K32
+#define HFX_BOYS_K32 5
 #endif
 
 // Contracted integrals of the shell quartet (bra | ket), slice (sa, sc):
@@ -255,7 +272,7 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const BP* __restrict_
             const PT T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
             const PT pref = PT(P.Kp) * kq * isq;
             PT u[NR], w[NR];
-            if constexpr (sizeof(PT) == 4 && HFX_FP32_RULES) {
+            if constexpr (sizeof(PT) == 4 && (HFX_FP32_RULES == 1 || (HFX_FP32_RULES == 2 && NR >= 3))) {
                 if constexpr (NR <= 2)
                     rys_roots_small_f<NR>(T, u, w, tab.boysf);
                 else
@@ -264,7 +281,7 @@ HFX_HD HFX_INLINE void eri_quartet_k(const ShellPair& bra, const BP* __restrict_
                 // FP32 quartets: the Boys function and the Rys rule in FP64
                 double ud[NR], wd[NR];
                 if constexpr (NR <= 2)
-                    rys_roots_small<NR>(double(T), ud, wd, tab.boys);
+                    rys_roots_small<NR, HFX_BOYS_K32>(double(T), ud, wd, tab.boys);
                 else
                     rys_roots<NR>(double(T), ud, wd, tab);
 #pragma unroll

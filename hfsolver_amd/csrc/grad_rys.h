@@ -284,10 +284,22 @@ HFX_HD HFX_INLINE void grad_quartet_k(const ShellPair& bra, const PrimPair* __re
             const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             const double pref = HFX_TWO_PI_52 * P.K * double(Qp.K) * ip * iq * isq;
             double u[NR], w[NR];
-            if constexpr (NR <= 2)
+            if constexpr (sizeof(R) == 4 && NR <= 2) {
+                rys_roots_small<NR, HFX_BOYS_K32>(T, u, w, tab.boys);
+            } else if constexpr (sizeof(R) == 4 && HFX_FP32_RULES >= 1) {
+                // single-precision derivative integrals: the rules of the FP32 J/K kernels
+                float uf[NR], wf[NR];
+                rys_roots_f<NR>(float(T), uf, wf, tab);
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    u[r] = uf[r];
+                    w[r] = wf[r];
+                }
+            } else if constexpr (NR <= 2) {
                 rys_roots_small<NR>(T, u, w, tab.boys);
-            else
+            } else {
                 rys_roots<NR>(T, u, w, tab);
+            }
             const double PA[3] = {P.PAx, P.PAy, P.PAz};
             const double QC[3] = {double(Qp.QCx), double(Qp.QCy), double(Qp.QCz)};
 #pragma unroll

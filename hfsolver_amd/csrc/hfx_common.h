@@ -182,9 +182,40 @@ HFX_HD HFX_INLINE ShellPair hfx_ld_pair(const ShellPair* p) {
 #endif
 }
 
-// Fields of a ket primitive pair in a field-major ket table: p, 1/p, P, P - C,
-// K, and the exponent of the C primitive (derivative integrals).
+// Fields of a ket primitive pair in a ket table: p, 1/p, P, P - C, K, and the
+// exponent of the C primitive (derivative integrals), in this order.
 static constexpr int KETPRIM_FIELDS = 10;
+
+// Per primitive k of a ket sublist the table holds 10 planes of `pitch` words
+// (pitch = the sublist's ket count rounded up to a multiple of 4): the first
+// eight fields (p, 1/p, P, P - C) in 16-byte chunks of two doubles or four
+// floats, chunk h of ket j at word (10 k pitch) + (h pitch + j) * PER, then K
+// and the C exponent as plain planes at words (10 k + 8) pitch + j and
+// (10 k + 9) pitch + j. Consecutive kets (consecutive lanes) read consecutive
+// words, and the energy kernels load a primitive with four (FP64) or two (FP32)
+// 16-byte loads and one 8- or 4-byte load per lane.
+template <typename T>
+struct KetChunk {
+    static constexpr int PER = 16 / (int)sizeof(T);   // fields per chunk
+    static constexpr int NCH = 8 / PER;               // chunks of the first eight fields
+};
+HFX_HD HFX_INLINE int ket_pitch(int nks) { return (nks + 3) & ~3; }
+struct alignas(16) KetChunkD {
+    double v[2];
+};
+struct alignas(16) KetChunkF {
+    float v[4];
+};
+template <typename T>
+struct KetChunkOf;
+template <>
+struct KetChunkOf<double> {
+    using type = KetChunkD;
+};
+template <>
+struct KetChunkOf<float> {
+    using type = KetChunkF;
+};
 
 
 static constexpr double HFX_PI = 3.14159265358979323846;

@@ -515,18 +515,27 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
     T.nd = ncart(pair_class_lb(c));
     T.ncomp = ncart(pair_class_la(c)) * T.nd;
     int nk = 0;
-    long long ps = 0;
+    long long ps = 0, psf = 0;
     for (int s = 0; s < S; s++) {
         t.off[s] = nk;
         t.np[s] = np_bucket(h->pairs[subl[s][0]].np);
         t.psoff[s] = ps;
+        t.psoff_f[s] = psf;
         nk += (int)subl[s].size();
-        ps += (long long)t.np[s] * KETPRIM_FIELDS * (long long)subl[s].size();
+        ps += (long long)t.np[s] * KETPRIM_FIELDS * (long long)ket_pitch((int)subl[s].size());
+        psf += (long long)t.np[s] * KETPRIM_FIELDS * (long long)ket_pitch((int)subl[s].size());
     }
     t.off[S] = nk;
     t.nk = nk;
     std::vector<KetInfo> info(nk);
-    std::vector<double> cd(3 * (size_t)nk), prim((size_t)ps);
+    std::vector<double> cd(3 * (size_t)nk), prim((size_t)ps, 0.0);
+    std::vector<float> primf((size_t)psf, 0.0f);
+    // word of field fi of primitive i of ket j in a sublist table of plane length pitch (KetChunk)
+    auto word = [](int fi, int i, int j, int pitch, int per) -> size_t {
+        const size_t pk = (size_t)i * KETPRIM_FIELDS * pitch;
+        if (fi < 8) return pk + ((size_t)(fi / per) * pitch + j) * per + fi % per;
+        return pk + (size_t)fi * pitch + j;
+    };
     std::vector<int> shcd(nk);
     std::vector<float> qk(nk);
     for (int s = 0; s < S; s++) {
@@ -546,12 +555,14 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
                 const PrimPair& q = h->prims[pi];
                 const double K = i < sp.np ? q.K : 0.0;
                 const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, K, h->pexp[pi].aa};
-                for (int fi = 0; fi < KETPRIM_FIELDS; fi++)
-                    prim[(size_t)t.psoff[s] + ((size_t)i * KETPRIM_FIELDS + fi) * nks + j] = f[fi];
+                const int pitch = ket_pitch(nks);
+                for (int fi = 0; fi < KETPRIM_FIELDS; fi++) {
+                    prim[(size_t)t.psoff[s] + word(fi, i, j, pitch, KetChunk<double>::PER)] = f[fi];
+                    primf[(size_t)t.psoff_f[s] + word(fi, i, j, pitch, KetChunk<float>::PER)] = (float)f[fi];
+                }
             }
         }
     }
-    std::vector<float> primf(prim.begin(), prim.end());
     KetInfo* d_info = nullptr;
     double *d_cd = nullptr, *d_prim = nullptr;
     float* d_primf = nullptr;

@@ -48,8 +48,8 @@ struct alignas(16) KetInfo {
 struct KetTab {
     const KetInfo* info;
     const double* cd;     // [3][nk]: C - D
-    const double* prim;   // sublist s at prim + psoff[s]: [np_s][KETPRIM_FIELDS][nk_s]
-    const float* primf;   // the same in single precision (FP32 kernels)
+    const double* prim;   // sublist s at prim + psoff[s]: [np_s][10 planes][pitch] (KetChunk)
+    const float* primf;   // the same in single precision (FP32 kernels), sublist s at primf + psoff_f[s]
     double* Dcd;          // [NC * ND][nk]: density blocks D_cd, gathered per build
     double* Jcd;          // [NC * ND][nk]: J_cd accumulators, scattered into J after the build
     const int* shcd;      // [nk]: shells C | D << 16 (density-weighted screening)
@@ -59,6 +59,7 @@ struct KetTab {
     int off[HFX_SUB_MAX + 1];
     int np[HFX_SUB_MAX];
     long long psoff[HFX_SUB_MAX];
+    long long psoff_f[HFX_SUB_MAX];
 };
 
 // Arguments of one row-mode J/K launch (bra class X, ket class Y, X >= Y):

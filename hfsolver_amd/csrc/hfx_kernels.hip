@@ -215,7 +215,7 @@ void jk_row_kernel(RowArgs a) {
         // ket primitive planes in the arithmetic type of the kernel
         const R* sprim;
         if constexpr (sizeof(R) == 4)
-            sprim = kt.primf + kt.psoff[s];
+            sprim = kt.primf + kt.psoff_f[s];
         else
             sprim = kt.prim + kt.psoff[s];
         // uniform trip count: every lane of a wavefront reaches the reductions
@@ -237,7 +237,7 @@ void jk_row_kernel(RowArgs a) {
             }
             const int cc = (ki.cp & 0x7fff) + sc, cd = a.offd + ((ki.cp >> 15) & 0x7fff);
             const double sket = (ki.cp >> 30) & 1 ? 0.5 : 1.0;
-            const KetPrimsSoA<R> kp{sprim + (col - soff), npk, nks};
+            const KetPrimsSoA<R> kp{sprim, col - soff, npk, ket_pitch(nks)};
             R Jcd[NCt][ND];
 #pragma unroll
             for (int c = 0; c < NCt; c++)
@@ -468,7 +468,7 @@ void grad_row_kernel(GradRowArgs a) {
             }
             const int cc = (ki.cp & 0x7fff) + sc, cd = a.offd + ((ki.cp >> 15) & 0x7fff);
             const double sket = (ki.cp >> 30) & 1 ? 0.5 : 1.0;
-            const KetPrimsSoA<double> kp{sprim + (col - soff), npk, nks};
+            const KetPrimsSoA<double> kp{sprim, col - soff, npk, ket_pitch(nks)};
             double gcd[6] = {0, 0, 0, 0, 0, 0};   // C, D components summed over the rows
             for (int r = 0; r < nr; r++) {
                 const int row = row0 + r;

@@ -112,23 +112,24 @@ HFX_HD HFX_INLINE double hfx_rcp(double x) {
 // u_2 = E / D and u_1 = B / E, the weights w_1 = (F_1 - u_2 F_0)/(u_1 - u_2)
 // = E (F_1 D - E F_0) / (B D - E^2) and w_2 = F_0 - w_1. The three quotients
 // share one reciprocal, 1 / (D E (B D - E^2)).
-template <int NR>
+template <int NR, int K = BOYS_K>
 HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const double* boys) {
     static_assert(NR == 1 || NR == 2, "closed form for one or two roots only");
+    static_assert(K >= 2 && K <= BOYS_K, "Taylor terms beyond the table");
     constexpr int NM = 2 * NR;  // moments F_0 .. F_{2NR-1}
     double F[NM];
     if (T < BOYS_TMAX) {
         const unsigned i = (unsigned)(T * BOYS_INV_DT + 0.5);
         const double x = i * BOYS_DT - T;
-        double xk[BOYS_K];
+        double xk[K];
 #pragma unroll
-        for (int k = 1; k < BOYS_K; k++) xk[k] = x * (1.0 / k);
+        for (int k = 1; k < K; k++) xk[k] = x * (1.0 / k);
         const double* row = boys + hfx_umul24(i, BOYS_NCOL);
 #pragma unroll
         for (int m = 0; m < NM; m++) {
-            double acc = row[m + BOYS_K - 1];
+            double acc = row[m + K - 1];
 #pragma unroll
-            for (int k = BOYS_K - 2; k >= 0; k--) acc = row[m + k] + acc * xk[k + 1];
+            for (int k = K - 2; k >= 0; k--) acc = row[m + k] + acc * xk[k + 1];
             F[m] = acc;
         }
         if constexpr (NR == 1) {
