@@ -27,6 +27,31 @@ def build_sanitized():
                     "-fno-omit-frame-pointer", SRC, SAN_SRC, "-o", SAN_EXE], check=True)
 
 
+ROOT = os.path.dirname(HERE)
+HOST_SAN_SRC = os.path.join(HERE, "csrc", "host_san.cpp")
+HOST_SAN_EXE = os.path.join(HERE, "csrc", "host_san")
+HOST_SAN_SUPP = os.path.join(HERE, "csrc", "lsan_rocm.supp")
+
+
+def build_host_sanitized():
+    """The engine's host runtime (hfsolver_amd/csrc/hfx_host.cpp) and the driver
+    host_san.cpp built with AddressSanitizer and UndefinedBehaviorSanitizer (host
+    code only, sanitizer runtimes linked statically), linked with the gfx950
+    kernel objects of the library build (hfsolver_amd/_build). Runs on a GPU."""
+    kobj = [os.path.join(ROOT, "hfsolver_amd", "_build", "hfx_kernels_%d.o" % p) for p in range(1, 8)]
+    missing = [o for o in kobj if not os.path.exists(o)]
+    if missing:
+        raise RuntimeError("kernel objects missing (python -m hfsolver_amd.build first): %s" % missing[0])
+    san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
+    host_o = os.path.join(HERE, "csrc", "host_san_hfx_host.o")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fopenmp", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"] + san +
+                   ["-c", os.path.join(ROOT, "hfsolver_amd", "csrc", "hfx_host.cpp"), "-o", host_o], check=True)
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-I" + os.path.join(ROOT, "include")] + san +
+                   ["-static-libasan", "-static-libubsan", HOST_SAN_SRC, host_o] + kobj +
+                   ["-fopenmp", "-L/opt/rocm/lib", "-lamdhip64", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib",
+                    "-o", HOST_SAN_EXE], check=True)
+
+
 def load():
     if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
         build()
