@@ -64,6 +64,12 @@ BOYS_K = 7
 BOYS_MMAX = 3                      # highest order needed (two roots: F_0..F_3)
 BOYS_NCOL = BOYS_MMAX + BOYS_K     # F_0 .. F_9 per grid point
 BOYS_NPT = int(BOYS_TMAX / BOYS_DT) + 1
+# Compact one-root table on the same grid: per grid point F_top(T0) and e^{-T0}
+# for top = BOYS_K (the FP64 kernels' Taylor terms) and top = BOYS1_KF (the
+# FP32 kernels'), so that a one-root rule reads 16 bytes and recovers
+# F_{top-1}(T0) .. F_0(T0) by the (stable) downward recursion
+# F_m = (2 T0 F_{m+1} + e^{-T0}) / (2m + 1).
+BOYS1_KF = 5
 # single-precision copy on the same grid for the FP32 kernels (used only by a
 # build with HFX_FP32_RULES=1): BOYSF_K terms, truncation < (1/64)^5/5! ~ 8e-12
 # relative, below float rounding
@@ -184,6 +190,8 @@ def main():
            "#define BOYS_NCOL %d" % BOYS_NCOL,
            "#define BOYS_NPT %d" % BOYS_NPT,
            "#define BOYS_SIZE %d" % (BOYS_NCOL * BOYS_NPT),
+           "#define BOYS1_KF %d" % BOYS1_KF,
+           "#define BOYS1_SIZE %d" % (4 * BOYS_NPT),
            "#define BOYSF_K %d" % BOYSF_K,
            "#define BOYSF_NCOL %d" % BOYSF_NCOL,
            "#define BOYSF_SIZE %d" % (BOYSF_NCOL * BOYS_NPT)]
@@ -225,6 +233,14 @@ def main():
     fms = [boys_moments(mp.mpf(i) * mp.mpf(BOYS_DT), BOYS_NCOL - 1) for i in range(BOYS_NPT)]
     for fm in fms:
         lines.append("  " + ", ".join("%.17e" % float(v) for v in fm) + ",")
+    lines.append("};")
+    # compact one-root table: BOYS1_TAB[4 i .. 4 i + 3] = F_{BOYS_K}(T0), e^{-T0}, F_{BOYS1_KF}(T0), e^{-T0}
+    lines.append("RYS_TABLE_QUALIFIER double BOYS1_TAB[BOYS1_SIZE] = {")
+    for i in range(BOYS_NPT):
+        T0 = mp.mpf(i) * mp.mpf(BOYS_DT)
+        top = boys_moments(T0, max(BOYS_K, BOYS1_KF))
+        e = mp.exp(-T0)
+        lines.append("  %.17e, %.17e, %.17e, %.17e," % (float(top[BOYS_K]), float(e), float(top[BOYS1_KF]), float(e)))
     lines.append("};")
     # its single-precision copy: BOYSF_TAB[i * BOYSF_NCOL + m] = F_m(i * BOYS_DT), m < BOYSF_NCOL
     lines.append("RYS_TABLE_QUALIFIER float BOYSF_TAB[BOYSF_SIZE] = {")

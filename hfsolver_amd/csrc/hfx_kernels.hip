@@ -37,7 +37,7 @@ namespace {
 
 __device__ HFX_INLINE RysTables dev_tables(const double* boys = hfx_dev::BOYS_TAB) {
     return RysTables{hfx_dev::RYS_POLY, hfx_dev::RYS_HERM_U, hfx_dev::RYS_HERM_W, boys, hfx_dev::BOYSF_TAB,
-                     hfx_dev::RYS_CHEBF};
+                     hfx_dev::RYS_CHEBF, hfx_dev::BOYS1_TAB};
 }
 
 // v of lane i - k within its row of 16 lanes (CTRL = row_shr:k), or of the

@@ -23,6 +23,7 @@ struct RysTables {
     const double* boys;  // BOYS_TAB (global memory, read through the caches)
     const float* boysf;  // BOYSF_TAB, single-precision Boys table of the FP32 kernels
     const float* chebf;  // RYS_CHEBF, single-precision truncated Chebyshev series of the FP32 kernels
+    const double* boys1; // BOYS1_TAB, compact one-root table (F_top and e^{-T0} per grid point)
 };
 
 // 1/sqrt(x) for positive normal x (p + q, or T beyond the tables): on the device
@@ -167,6 +168,44 @@ HFX_HD HFX_INLINE void rys_roots_small(double T, double* u, double* w, const dou
         const double w1 = (m1 * D - E * m0) * E * D * E * r;
         w[0] = w1;
         w[1] = m0 - w1;
+    }
+}
+
+// One-root rule from the compact table BOYS1_TAB: per grid point T0 the
+// moment F_K(T0) and e^{-T0} in one 16-byte load (K = BOYS_K for the FP64
+// kernels, BOYS1_KF for the FP32 kernels, which read the second half of the
+// row); F_{K-1}(T0) .. F_0(T0) by the downward recursion
+// F_m = (2 T0 F_{m+1} + e^{-T0}) / (2m + 1), stable in this direction; then
+// F_0(T) and F_1(T) by the K-term Taylor series of rys_roots_small, and
+// u = F_1 / F_0, w = F_0.
+template <int K>
+HFX_HD HFX_INLINE void rys_roots_one(double T, double* u, double* w, const double* boys1) {
+    static_assert(K == BOYS_K || K == BOYS1_KF, "the table holds F_K for these K only");
+    if (T < BOYS_TMAX) {
+        const unsigned i = (unsigned)(T * BOYS_INV_DT + 0.5);
+        const double T0 = i * BOYS_DT;
+        const double x = T0 - T;
+        const double* row = boys1 + 4 * i + (K == BOYS_K ? 0 : 2);
+        const double e = row[1];
+        double Fm[K + 1];
+        Fm[K] = row[0];
+#pragma unroll
+        for (int m = K - 1; m >= 0; m--) Fm[m] = fma(2.0 * T0, Fm[m + 1], e) * (1.0 / (2 * m + 1));
+        double xk[K];
+#pragma unroll
+        for (int k = 1; k < K; k++) xk[k] = x * (1.0 / k);
+        double f0 = Fm[K - 1], f1 = Fm[K];
+#pragma unroll
+        for (int k = K - 2; k >= 0; k--) {
+            f0 = Fm[k] + f0 * xk[k + 1];
+            f1 = Fm[k + 1] + f1 * xk[k + 1];
+        }
+        u[0] = f1 * hfx_rcp(f0);
+        w[0] = f0;
+    } else {
+        const double rt = hfx_rsqrt(T);
+        u[0] = 0.5 * rt * rt;
+        w[0] = 0.5 * HFX_SQRT_PI * rt;
     }
 }
 

@@ -16,6 +16,8 @@
 #define BOYS_NCOL 10
 #define BOYS_NPT 1473
 #define BOYS_SIZE 14730
+#define BOYS1_KF 5
+#define BOYS1_SIZE 5892
 #define BOYSF_K 5
 #define BOYSF_NCOL 8
 #define BOYSF_SIZE 11784

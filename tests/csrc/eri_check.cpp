@@ -17,7 +17,7 @@ namespace {
 
 RysTables tabs() {
     return RysTables{chk_tables::RYS_POLY, chk_tables::RYS_HERM_U, chk_tables::RYS_HERM_W, chk_tables::BOYS_TAB,
-                     chk_tables::BOYSF_TAB, chk_tables::RYS_CHEBF};
+                     chk_tables::BOYSF_TAB, chk_tables::RYS_CHEBF, chk_tables::BOYS1_TAB};
 }
 
 void make_pair(int la, int lb, const double* A, const double* B, int na, const double* aa, const double* ca, int nb,
@@ -203,6 +203,10 @@ double chk_rys(int nr, double T, double* u, double* w) {
             }
             break;
         }
+        case -31: rys_roots_one<BOYS_K>(T, u, w, chk_tables::BOYS1_TAB); break;     // compact one-root table
+        case -41: rys_roots_one<BOYS1_KF>(T, u, w, chk_tables::BOYS1_TAB); break;   // ... with the FP32 kernels' terms
+        case -51: rys_roots_small<1, HFX_BOYS_K32>(T, u, w, chk_tables::BOYS_TAB); break;   // FP32 kernels' terms
+        case -52: rys_roots_small<2, HFX_BOYS_K32>(T, u, w, chk_tables::BOYS_TAB); break;
         case 1: rys_roots<1>(T, u, w, tabs()); break;
         case 2: rys_roots<2>(T, u, w, tabs()); break;
         case 3: rys_roots<3>(T, u, w, tabs()); break;

@@ -106,6 +106,28 @@ def test_rys_rule_reproduces_boys_moments(lib):
                     assert abs(np.sum(w[:nr] * u[:nr] ** k) - Ff[k]) <= 2e-7 * Ff[0] + 1e-6 * Ff[k], (nr, T, k)
 
 
+def test_one_root_recursion_and_short_taylor_rules(lib):
+    """The compact one-root table (F_top(T0), e^{-T0}; downward recursion to
+    F_0) with the FP64 kernels' Taylor terms matches the oracle's Boys
+    moments as tightly as the full table; with the FP32 kernels' five terms,
+    and the five-term closed forms of one and two roots (the FP32 kernels'
+    FP64 rules), within the truncation bound (1/64)^5/5! < 8e-12 (two roots:
+    the root formulas amplify it about tenfold), at every table interval's
+    worst offset and beyond the table."""
+    lib.chk_rys.restype = ctypes.c_double
+    u = np.zeros(5)
+    w = np.zeros(5)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    Ts = [(k + 0.5) / 32 - 1e-12 for k in range(46 * 32)] + [0.0, 1e-9, 45.99, 46.0, 60.0, 1e3, 1e9]
+    for code, nr, rel in ((-31, 1, 1e-15), (-41, 1, 2e-11), (-51, 1, 2e-11), (-52, 2, 2e-10)):
+        for T in Ts:
+            lib.chk_rys(code, ctypes.c_double(T), p(u), p(w))
+            F = O.boys(2 * nr - 1, T)
+            for k in range(2 * nr):
+                assert abs(np.sum(w[:nr] * u[:nr] ** k) - F[k]) <= rel * F[0] + 8 * rel * F[k], (code, T, k)
+            assert np.all(np.diff(u[:nr]) > 0) and np.all(u[:nr] > 0) and np.all(u[:nr] < 1)
+
+
 def test_derivative_quartets_match_oracle(lib):
     """grad_rys.h instantiated on the host: sum gam_abcd d(ab|cd)/d(A, B, C) for
     every class combination against the oracle's derivative integrals
