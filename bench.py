@@ -366,11 +366,11 @@ def main():
             Em, gm = ffm(x)
             torch.cuda.synchronize()
             tm.append(time.perf_counter() - t0)
-            E64, g64 = ff64(x)
-            d = np.asarray(gm) - np.asarray(g64)
+            Ef64, gf64 = ff64(x)
+            d = np.asarray(gm) - np.asarray(gf64)
             gmax.append(float(np.abs(d).max()))
             grms.append(float(np.sqrt(np.mean(d * d))))
-            dEs.append(abs(Em - E64))
+            dEs.append(abs(Em - Ef64))
             its.append(int(ffm.last.iterations))
             log("  forces %d: mixed SCF+gradient %.2f s (%d SCF iterations), |dE| %.2e Eh, max|dg| %.2e, rms %.2e Eh/bohr"
                 % (k, tm[-1], its[-1], dEs[-1], gmax[-1], grms[-1]))

@@ -187,6 +187,10 @@ int fail(hfx_t* h, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(h, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+double seconds_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
 int dmalloc(hfx_t* h, void** p, size_t bytes, bool geom) {
     if (bytes == 0) bytes = 8;
     hipError_t e = hipMalloc(p, bytes);
@@ -609,6 +613,7 @@ int build_lists(hfx_t* h) {
     std::vector<int> pos(h->pairs.size(), -1);
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
         for (int i = 0; i < (int)h->cls_list[c].size(); i++) pos[h->cls_list[c][i]] = i;
+    const auto tk = std::chrono::steady_clock::now();
     // ket tables: sublists of equal primitive-pair count, each still Q-descending
     std::vector<std::vector<int>> subl[HFX_NPAIRCLASS];
     for (int c = 0; c < HFX_NPAIRCLASS; c++) {
@@ -625,6 +630,7 @@ int build_lists(hfx_t* h) {
         }
         if (build_ket_table(h, c, subl[c])) return 1;
     }
+    if (h->opts.verbose) fprintf(stderr, "hfx: class lists and ket tables %.3f s\n", seconds_since(tk));
     for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
         for (int ck = 0; ck <= cb; ck++) {
             const std::vector<int>& bl = h->cls_list[cb];
@@ -863,9 +869,11 @@ int build_lists(hfx_t* h) {
 }
 
 int setup_geometry(hfx_t* h) {
+    const auto t0 = std::chrono::steady_clock::now();
     free_geometry(h);
     if (upload(h, &h->d_xyz, h->xyz, true)) return 1;
     build_pairs(h);
+    const double t_pairs = seconds_since(t0);
     if (upload(h, &h->d_pairs, h->pairs, true)) return 1;
     if (upload(h, &h->d_prims, h->prims, true)) return 1;
     {
@@ -878,9 +886,15 @@ int setup_geometry(hfx_t* h) {
         if (upload(h, &h->d_primsf, pf, true)) return 1;
     }
     if (upload(h, &h->d_pexp, h->pexp, true)) return 1;
+    const double t_upload = seconds_since(t0);
     if (compute_schwarz(h)) return 1;
     HIPCHK(h, hipMemcpy(h->d_pairs, h->pairs.data(), h->pairs.size() * sizeof(ShellPair), hipMemcpyHostToDevice));
-    return build_lists(h);
+    const double t_schwarz = seconds_since(t0);
+    const int rc = build_lists(h);
+    if (h->opts.verbose)
+        fprintf(stderr, "hfx: geometry set-up %.3f s: pairs %.3f, uploads %.3f, Schwarz bounds %.3f, lists %.3f\n",
+                seconds_since(t0), t_pairs, t_upload - t_pairs, t_schwarz - t_upload, seconds_since(t0) - t_schwarz);
+    return rc;
 }
 
 // ------------------------------------------------------ one-electron ----
