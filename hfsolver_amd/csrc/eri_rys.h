@@ -189,7 +189,8 @@ struct KetPrimsSoA {
 #pragma unroll
             for (int e = 0; e < PER; e++) f[h * PER + e] = c.v[e];
         }
-        return KetPrimT<T>{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], pk[(size_t)8 * pitch + j]};
+        // slots: p, 1/p, Px, Py, Pz, K, QCx, QCy | QCz (plane 8)
+        return KetPrimT<T>{f[0], f[1], f[2], f[3], f[4], f[6], f[7], pk[(size_t)8 * pitch + j], f[5]};
     }
     // exponent of the C primitive of primitive pair k
     HFX_HD HFX_INLINE T cexp(int k) const { return tab[((size_t)k * KETPRIM_FIELDS + 9) * pitch + j]; }

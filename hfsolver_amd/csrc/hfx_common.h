@@ -187,13 +187,16 @@ HFX_HD HFX_INLINE ShellPair hfx_ld_pair(const ShellPair* p) {
 static constexpr int KETPRIM_FIELDS = 10;
 
 // Per primitive k of a ket sublist the table holds 10 planes of `pitch` words
-// (pitch = the sublist's ket count rounded up to a multiple of 4): the first
-// eight fields (p, 1/p, P, P - C) in 16-byte chunks of two doubles or four
-// floats, chunk h of ket j at word (10 k pitch) + (h pitch + j) * PER, then K
-// and the C exponent as plain planes at words (10 k + 8) pitch + j and
-// (10 k + 9) pitch + j. Consecutive kets (consecutive lanes) read consecutive
-// words, and the energy kernels load a primitive with four (FP64) or two (FP32)
-// 16-byte loads and one 8- or 4-byte load per lane.
+// (pitch = the sublist's ket count rounded up to a multiple of 4), the fields
+// stored in the slot order p, 1/p, P, K, (P - C)_x, (P - C)_y | (P - C)_z,
+// C exponent: slots 0-7 in 16-byte chunks of two doubles or four floats,
+// chunk h of ket j at word (10 k pitch) + (h pitch + j) * PER, slots 8 and 9
+// as plain planes at words (10 k + 8) pitch + j and (10 k + 9) pitch + j.
+// Consecutive kets (consecutive lanes) read consecutive words; an energy
+// kernel loads a primitive with four (FP64) or two (FP32) 16-byte loads and
+// one plain load per lane, and a class whose ket is an s-s pair (no P - C)
+// with three (FP64) or two (FP32) 16-byte loads.
+static constexpr int KETPRIM_SLOT[KETPRIM_FIELDS] = {0, 1, 2, 3, 4, 6, 7, 8, 5, 9};   // slot of each field
 template <typename T>
 struct KetChunk {
     static constexpr int PER = 16 / (int)sizeof(T);   // fields per chunk

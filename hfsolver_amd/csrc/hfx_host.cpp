@@ -534,7 +534,7 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
     std::vector<KetInfo> info(nk);
     std::vector<double> cd(3 * (size_t)nk), prim((size_t)ps, 0.0);
     std::vector<float> primf((size_t)psf, 0.0f);
-    // word of field fi of primitive i of ket j in a sublist table of plane length pitch (KetChunk)
+    // word of slot fi of primitive i of ket j in a sublist table of plane length pitch (KetChunk)
     auto word = [](int fi, int i, int j, int pitch, int per) -> size_t {
         const size_t pk = (size_t)i * KETPRIM_FIELDS * pitch;
         if (fi < 8) return pk + ((size_t)(fi / per) * pitch + j) * per + fi % per;
@@ -561,8 +561,9 @@ int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) 
                 const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, K, h->pexp[pi].aa};
                 const int pitch = ket_pitch(nks);
                 for (int fi = 0; fi < KETPRIM_FIELDS; fi++) {
-                    prim[(size_t)t.psoff[s] + word(fi, i, j, pitch, KetChunk<double>::PER)] = f[fi];
-                    primf[(size_t)t.psoff_f[s] + word(fi, i, j, pitch, KetChunk<float>::PER)] = (float)f[fi];
+                    const int slot = KETPRIM_SLOT[fi];
+                    prim[(size_t)t.psoff[s] + word(slot, i, j, pitch, KetChunk<double>::PER)] = f[fi];
+                    primf[(size_t)t.psoff_f[s] + word(slot, i, j, pitch, KetChunk<float>::PER)] = (float)f[fi];
                 }
             }
         }
