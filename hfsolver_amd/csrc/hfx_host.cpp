@@ -645,6 +645,28 @@ int build_lists(hfx_t* h) {
                 rke[prec].assign((size_t)nb * S, 0);
             }
             size_t first_combo = h->combos.size();
+            // row mode when the K rows of the bra shells fit in LDS
+            hfx_engine::RowCombo rc;
+            rc.cb = cb;
+            rc.ck = ck;
+            const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
+            rc.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
+            rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
+            // the FP64 and FP32 launches slice differently: plan for the larger LDS of the two
+            int nrow64 = 0, nab64 = 0, nrow32 = 0, nab32 = 0;
+            hfx_row_shape(cb, ck, 0, &nrow64, &nab64);
+            hfx_row_shape(cb, ck, 1, &nrow32, &nab32);
+            auto row_lds = [&](int R, int dl) {
+                return std::max(hfx_row_lds(nrow64, nab64, rc.ncols, R, dl, false),
+                                hfx_row_lds(nrow32, nab32, rc.ncols, R, dl, true));
+            };
+            int dlds = h->row_dlds;
+            if (dlds && row_lds(1, 1) > std::min(h->lds_budget, h->dlds_budget)) dlds = 0;
+            const size_t need = row_lds(1, dlds);
+            // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
+            int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
+            if (R == 0 && need <= h->lds_budget) R = 1;
+            const bool rowmode = h->opts.kernel_mode == 0 && R >= 1 && S <= HFX_SUB_MAX && rc.ncols < 32768;
             for (int s = 0; s < S; s++) {
                 const std::vector<int>& kl = subl[ck][s];
                 const int npk = np_bucket(h->pairs[kl[0]].np);
@@ -679,7 +701,9 @@ int build_lists(hfx_t* h) {
                 cmb.ck = ck;
                 cmb.nket = nk;
                 cmb.npk = npk;
-                if (upload(h, &cmb.d_ket, kl, true)) return 1;
+                // the tile-mode lists (ket list, tiles, per-row ranges, gradient blocks)
+                // are built only for combinations that do not run in row mode
+                if (!rowmode && upload(h, &cmb.d_ket, kl, true)) return 1;
                 for (int prec = 0; prec < 2; prec++) {
                     std::vector<int> kb(nb), ke(nb), ti0, tj0;
                     for (int i = 0; i < nb; i++) {
@@ -690,6 +714,10 @@ int build_lists(hfx_t* h) {
                         if (ke[i] > kb[i]) cmb.nprimq[prec] += h->pairs[bl[i]].np * (cnp[ke[i]] - cnp[kb[i]]);
                         rkb[prec][(size_t)i * S + s] = soff + kb[i];
                         rke[prec][(size_t)i * S + s] = soff + std::max(kb[i], ke[i]);
+                    }
+                    if (rowmode) {
+                        cmb.ntiles[prec] = cmb.nq[prec] > 0 ? 1 : 0;   // marks quartets for the statistics
+                        continue;
                     }
                     for (int r0 = 0; r0 < nb; r0 += HFX_TILE_ROWS) {
                         int lo = 1 << 30, hi = 0;
@@ -734,28 +762,6 @@ int build_lists(hfx_t* h) {
                 }
                 if (cmb.ntiles[0] + cmb.ntiles[1] > 0) h->combos.push_back(cmb);
             }
-            // row mode when the K rows of the bra shells fit in LDS
-            hfx_engine::RowCombo rc;
-            rc.cb = cb;
-            rc.ck = ck;
-            const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
-            rc.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
-            rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
-            // the FP64 and FP32 launches slice differently: plan for the larger LDS of the two
-            int nrow64 = 0, nab64 = 0, nrow32 = 0, nab32 = 0;
-            hfx_row_shape(cb, ck, 0, &nrow64, &nab64);
-            hfx_row_shape(cb, ck, 1, &nrow32, &nab32);
-            auto row_lds = [&](int R, int dl) {
-                return std::max(hfx_row_lds(nrow64, nab64, rc.ncols, R, dl, false),
-                                hfx_row_lds(nrow32, nab32, rc.ncols, R, dl, true));
-            };
-            int dlds = h->row_dlds;
-            if (dlds && row_lds(1, 1) > std::min(h->lds_budget, h->dlds_budget)) dlds = 0;
-            const size_t need = row_lds(1, dlds);
-            // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
-            int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
-            if (R == 0 && need <= h->lds_budget) R = 1;
-            const bool rowmode = h->opts.kernel_mode == 0 && R >= 1 && S <= HFX_SUB_MAX && rc.ncols < 32768;
             for (size_t ci = first_combo; ci < h->combos.size(); ci++) h->combos[ci].rowmode = rowmode;
             if (!rowmode) continue;
             rc.R = R;
