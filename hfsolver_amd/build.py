@@ -3,8 +3,9 @@
     python -m hfsolver_amd.build            # library only
     python -m hfsolver_amd.build --check    # also the host check harness of tests/
 
-The kernels (csrc/hfx_kernels.hip) are compiled for gfx950 only, with packed
-FP32 instructions disabled; the host runtime (csrc/hfx_host.cpp) with OpenMP.
+The kernels (csrc/hfx_kernels.hip, in parts, and csrc/hfx_lists.hip, the
+quartet lists) are compiled for gfx950 only, with packed FP32 instructions
+disabled; the host runtime (csrc/hfx_host.cpp) with OpenMP.
 The Cython binding (_hfx.pyx over c_hfx.pxd) is compiled against the library
 into hfsolver_amd/_hfx<EXT_SUFFIX>.
 """
@@ -46,6 +47,8 @@ def build_library(verbose=True):
         objs.append(o)
         jobs.append(DEVICE_FLAGS + ["-DHFX_PART=%d" % part] + extra +
                     ["-c", os.path.join(CSRC, "hfx_kernels.hip"), "-o", o])
+    objs.append(os.path.join(BUILD, "hfx_lists.o"))
+    jobs.append(DEVICE_FLAGS + ["-c", os.path.join(CSRC, "hfx_lists.hip"), "-o", objs[-1]])
     objs.append(os.path.join(BUILD, "hfx_host.o"))
     jobs.append(HOST_FLAGS + ["-c", os.path.join(CSRC, "hfx_host.cpp"), "-o", objs[-1]])
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))

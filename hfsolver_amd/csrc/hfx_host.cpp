@@ -8,12 +8,13 @@
 //   2. all shell pairs (A,B), oriented l_A >= l_B, with their primitive pairs
 //      (negligible primitive pairs screened out);
 //   3. Schwarz bounds Q_AB = sqrt(max |(ab|ab)|) on the GPU;
-//   4. per pair class, the significant pairs sorted by Q (descending) and the
-//      class's field-major ket table (sublists by primitive count); for every
-//      class combination each bra row's range of kets per sublist with
-//      Q_ab Q_cd >= schwarz_threshold, split into an FP64 and an FP32 share at
-//      fp32_threshold, and the row-mode block list (tiles where the LDS rows do
-//      not fit).
+//   4. on the GPU (hfx_lists.hip): per pair class, the significant pairs
+//      sorted by Q (descending) and the class's chunked ket table (sublists by
+//      primitive count); for every class combination each bra row's range of
+//      kets per sublist with Q_ab Q_cd >= schwarz_threshold, split into an
+//      FP64 and an FP32 share at fp32_threshold, the quartet counts, and the
+//      row-mode block list; the host sizes the allocations from counts and
+//      builds tile lists only where the LDS rows do not fit.
 // Per Fock build (hfx_fock_2e): P -> scaled symmetric D, D_cd gathered per ket,
 // one launch per class combination and precision, J_cd scattered, then J, K
 // symmetrised and unscaled, G = J - K/2. One-electron integrals and their
@@ -23,6 +24,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -508,150 +510,143 @@ int np_bucket(int np) {
     return (np + 255) / 256 * 256;
 }
 
-// Field-major ket table of pair class c from its sublists (see KetTab).
-int build_ket_table(hfx_t* h, int c, const std::vector<std::vector<int>>& subl) {
+// Ket table of pair class c (see KetTab), filled on the GPU from the pairs in
+// (class, bucket) order: subs = its sublists' (bucket, count) in ascending
+// bucket order, sub0 = the class's first entry in that order. aux receives the
+// per-ket arrays of the range search (bound, class position, primitive counts
+// and their prefix sums).
+int build_ket_table(hfx_t* h, int c, const std::vector<std::array<int, 2>>& subs, int sub0, const PairOrder& po,
+                    KetAux& aux) {
     hfx_engine::KetTable& T = h->ktab[c];
     T = hfx_engine::KetTable{};
     KetTab& t = T.t;
-    const int S = (int)subl.size();
+    const int S = (int)subs.size();
     t.S = S;
-    if (S == 0 || S > HFX_SUB_MAX) return 0;   // no table: the class is a ket of tile-mode launches only
     T.nd = ncart(pair_class_lb(c));
     T.ncomp = ncart(pair_class_la(c)) * T.nd;
     int nk = 0;
-    long long ps = 0, psf = 0;
+    long long ps = 0;
     for (int s = 0; s < S; s++) {
         t.off[s] = nk;
-        t.np[s] = np_bucket(h->pairs[subl[s][0]].np);
+        t.np[s] = subs[s][0];
         t.psoff[s] = ps;
-        t.psoff_f[s] = psf;
-        nk += (int)subl[s].size();
-        ps += (long long)t.np[s] * KETPRIM_FIELDS * (long long)ket_pitch((int)subl[s].size());
-        psf += (long long)t.np[s] * KETPRIM_FIELDS * (long long)ket_pitch((int)subl[s].size());
+        t.psoff_f[s] = ps;
+        nk += subs[s][1];
+        ps += (long long)t.np[s] * KETPRIM_FIELDS * (long long)ket_pitch(subs[s][1]);
     }
     t.off[S] = nk;
     t.nk = nk;
-    std::vector<KetInfo> info(nk);
-    std::vector<double> cd(3 * (size_t)nk), prim((size_t)ps, 0.0);
-    std::vector<float> primf((size_t)psf, 0.0f);
-    // word of slot fi of primitive i of ket j in a sublist table of plane length pitch (KetChunk)
-    auto word = [](int fi, int i, int j, int pitch, int per) -> size_t {
-        const size_t pk = (size_t)i * KETPRIM_FIELDS * pitch;
-        if (fi < 8) return pk + ((size_t)(fi / per) * pitch + j) * per + fi % per;
-        return pk + (size_t)fi * pitch + j;
-    };
-    std::vector<int> shcd(nk);
-    std::vector<float> qk(nk);
-    for (int s = 0; s < S; s++) {
-        const int nks = (int)subl[s].size();
-        for (int j = 0; j < nks; j++) {
-            const ShellPair& sp = h->pairs[subl[s][j]];
-            const int k = t.off[s] + j;
-            if (sp.cpa >= 32768 || sp.cpb >= 32768) return fail(h, "ket table: too many functions of one angular momentum");
-            info[k] = KetInfo{subl[s][j], sp.fa, sp.fb, sp.cpa | (sp.cpb << 15) | ((sp.A == sp.B ? 1 : 0) << 30)};
-            shcd[k] = (sp.A & 0xffff) | (sp.B << 16);
-            qk[k] = (float)(sp.Q * (1.0 + 1e-6));
-            cd[k] = sp.ABx;
-            cd[(size_t)nk + k] = sp.ABy;
-            cd[2 * (size_t)nk + k] = sp.ABz;
-            for (int i = 0; i < t.np[s]; i++) {   // padding: last primitive pair with K = 0
-                const int pi = sp.p0 + std::min(i, sp.np - 1);
-                const PrimPair& q = h->prims[pi];
-                const double K = i < sp.np ? q.K : 0.0;
-                const double f[KETPRIM_FIELDS] = {q.p, q.ip, q.Px, q.Py, q.Pz, q.PAx, q.PAy, q.PAz, K, h->pexp[pi].aa};
-                const int pitch = ket_pitch(nks);
-                for (int fi = 0; fi < KETPRIM_FIELDS; fi++) {
-                    const int slot = KETPRIM_SLOT[fi];
-                    prim[(size_t)t.psoff[s] + word(slot, i, j, pitch, KetChunk<double>::PER)] = f[fi];
-                    primf[(size_t)t.psoff_f[s] + word(slot, i, j, pitch, KetChunk<float>::PER)] = (float)f[fi];
-                }
-            }
-        }
-    }
-    KetInfo* d_info = nullptr;
-    double *d_cd = nullptr, *d_prim = nullptr;
-    float* d_primf = nullptr;
-    if (upload(h, &d_info, info, true)) return 1;
-    if (upload(h, &d_cd, cd, true)) return 1;
-    if (upload(h, &d_prim, prim, true)) return 1;
-    if (upload(h, &d_primf, primf, true)) return 1;
-    int* d_shcd = nullptr;
-    float* d_qk = nullptr;
-    if (upload(h, &d_shcd, shcd, true)) return 1;
-    if (upload(h, &d_qk, qk, true)) return 1;
-    t.shcd = d_shcd;
-    t.qk = d_qk;
+    KetOut o{};
+    if (dmalloc(h, (void**)&o.info, (size_t)nk * sizeof(KetInfo), true)) return 1;
+    if (dmalloc(h, (void**)&o.cd, 3 * (size_t)nk * sizeof(double), true)) return 1;
+    if (dmalloc(h, (void**)&o.shcd, (size_t)nk * sizeof(int), true)) return 1;
+    if (dmalloc(h, (void**)&o.qk, (size_t)nk * sizeof(float), true)) return 1;
+    if (dmalloc(h, (void**)&o.prim, (size_t)ps * sizeof(double), true)) return 1;
+    if (dmalloc(h, (void**)&o.primf, (size_t)ps * sizeof(float), true)) return 1;
+    if (dmalloc(h, (void**)&aux.q, (size_t)nk * sizeof(double), true)) return 1;
+    if (dmalloc(h, (void**)&aux.pos, (size_t)nk * sizeof(int), true)) return 1;
+    if (dmalloc(h, (void**)&aux.np, (size_t)(nk + 1) * sizeof(int), true)) return 1;
+    if (dmalloc(h, (void**)&aux.cnp, (size_t)(nk + 1) * sizeof(int), true)) return 1;
     if (dmalloc(h, (void**)&t.Dcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
     if (dmalloc(h, (void**)&t.Jcd, (size_t)T.ncomp * nk * sizeof(double), true)) return 1;
-    t.info = d_info;
-    t.cd = d_cd;
-    t.prim = d_prim;
-    t.primf = d_primf;
+    // padding words of the primitive planes (kets beyond a sublist's count) stay zero
+    HIPCHK(h, hipMemsetAsync(o.prim, 0, (size_t)ps * sizeof(double), h->stream));
+    HIPCHK(h, hipMemsetAsync(o.primf, 0, (size_t)ps * sizeof(float), h->stream));
+    HIPCHK(h, hipMemsetAsync(aux.np + nk, 0, sizeof(int), h->stream));
+    t.info = o.info;
+    t.cd = o.cd;
+    t.prim = o.prim;
+    t.primf = o.primf;
+    t.shcd = o.shcd;
+    t.qk = o.qk;
+    HIPCHK(h, hfx_launch_ket_fill(t, o, aux, po.sub_order + sub0, h->d_pairs, h->d_prims, h->d_pexp, po.pos, h->stream));
     return 0;
 }
 
+// Screened quartet lists of the current geometry, built on the GPU
+// (hfx_lists.hip): class-sorted significant pairs, ket tables, per-row ket
+// ranges of every class combination with their quartet counts, row-mode block
+// lists. The host sizes the allocations from counts it reads back, plans the
+// row-mode LDS and, for class combinations outside row mode (kernel_mode = 1,
+// LDS rows that do not fit), builds the tile lists from the ranges.
 int build_lists(hfx_t* h) {
     const double thr = h->opts.schwarz_threshold;
     const double thr32 = h->opts.fp32_threshold;
-    double qmax = 0;
-    for (const ShellPair& p : h->pairs) qmax = std::max(qmax, p.Q);
-    h->npairs_sig = 0;
-    for (int c = 0; c < HFX_NPAIRCLASS; c++) h->cls_list[c].clear();
-    for (int i = 0; i < (int)h->pairs.size(); i++)
-        if (h->pairs[i].Q * qmax >= thr && h->pairs[i].Q > 0) {
-            h->cls_list[h->pairs[i].cls].push_back(i);
-            h->npairs_sig += 1;
-        }
+    const int np = (int)h->pairs.size();
     for (int c = 0; c < HFX_NPAIRCLASS; c++) {
-        std::stable_sort(h->cls_list[c].begin(), h->cls_list[c].end(),
-                         [&](int a, int b) { return h->pairs[a].Q > h->pairs[b].Q; });
-        if (!h->cls_list[c].empty())
-            if (upload(h, &h->d_cls_list[c], h->cls_list[c], true)) return 1;
+        h->cls_list[c].clear();
+        h->d_cls_list[c] = nullptr;
     }
     h->combos.clear();
     h->rcombos.clear();
-    // position of every significant pair in its class list
-    std::vector<int> pos(h->pairs.size(), -1);
-    for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        for (int i = 0; i < (int)h->cls_list[c].size(); i++) pos[h->cls_list[c][i]] = i;
-    const auto tk = std::chrono::steady_clock::now();
-    // ket tables: sublists of equal primitive-pair count, each still Q-descending
-    std::vector<std::vector<int>> subl[HFX_NPAIRCLASS];
+    h->npairs_sig = 0;
+    if (np == 0) return 0;
+    PairOrder po;
+    if (dmalloc(h, (void**)&po.cls_order, (size_t)np * sizeof(int), true)) return 1;
+    if (dmalloc(h, (void**)&po.sub_order, (size_t)np * sizeof(int), true)) return 1;
+    if (dmalloc(h, (void**)&po.pos, (size_t)np * sizeof(int), true)) return 1;
+    HIPCHK(h, hfx_pair_order(h->d_pairs, np, thr, po, h->stream));
+    h->npairs_sig = po.nsig;
+    // class lists: views of the sorted array on the device, a host copy for the counts and the tile lists
+    std::vector<int> order(po.nsig), hsub;
+    if (po.nsig > 0)
+        HIPCHK(h, hipMemcpy(order.data(), po.cls_order, (size_t)po.nsig * sizeof(int), hipMemcpyDeviceToHost));
+    int cls_off[HFX_NPAIRCLASS + 1] = {0};
     for (int c = 0; c < HFX_NPAIRCLASS; c++) {
-        if (h->cls_list[c].empty()) continue;
-        std::vector<int> nps;
-        for (int pid : h->cls_list[c]) nps.push_back(np_bucket(h->pairs[pid].np));
-        std::sort(nps.begin(), nps.end());
-        nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
-        for (int npk : nps) {
-            std::vector<int> kl;
-            for (int pid : h->cls_list[c])
-                if (np_bucket(h->pairs[pid].np) == npk) kl.push_back(pid);
-            subl[c].push_back(kl);
-        }
-        if (build_ket_table(h, c, subl[c])) return 1;
+        cls_off[c + 1] = cls_off[c] + po.ncls[c];
+        h->cls_list[c].assign(order.begin() + cls_off[c], order.begin() + cls_off[c + 1]);
+        if (po.ncls[c] > 0) h->d_cls_list[c] = po.cls_order + cls_off[c];
     }
-    if (h->opts.verbose) fprintf(stderr, "hfx: class lists and ket tables %.3f s\n", seconds_since(tk));
+    // sublists (bucket, count) of every class, in the (class, bucket) order
+    std::vector<std::array<int, 2>> subs[HFX_NPAIRCLASS];
+    int sub0[HFX_NPAIRCLASS] = {0};
+    {
+        int off = 0;
+        for (const auto& r : po.runs) {
+            if (subs[r[0]].empty()) sub0[r[0]] = off;
+            subs[r[0]].push_back({r[1], r[2]});
+            off += r[2];
+        }
+    }
+    KetAux aux[HFX_NPAIRCLASS] = {};
+    const auto tk = std::chrono::steady_clock::now();
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) {
+        if (subs[c].empty()) continue;
+        if ((int)subs[c].size() > HFX_SUB_MAX)
+            return fail(h, "pair class %d has %zu primitive-count sublists (at most %d)", c, subs[c].size(),
+                        HFX_SUB_MAX);
+        if (build_ket_table(h, c, subs[c], sub0[c], po, aux[c])) return 1;
+    }
+    if (h->opts.verbose) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        fprintf(stderr, "hfx: class lists and ket tables %.3f s\n", seconds_since(tk));
+    }
+    // per class combination: row-mode plan, then the ket ranges of every bra row on the GPU
+    struct Pending {
+        int cb, ck, nb, S, R, dlds;
+        bool rowmode;
+        size_t stat;          // first entry of its statistics
+        int* rkb[2];
+        int* rke[2];
+        hfx_engine::RowCombo rc;
+    };
+    std::vector<Pending> pend;
+    size_t nstat = 0;
     for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
         for (int ck = 0; ck <= cb; ck++) {
-            const std::vector<int>& bl = h->cls_list[cb];
-            if (bl.empty() || h->cls_list[ck].empty()) continue;
-            const int nb = (int)bl.size();
-            const int S = (int)subl[ck].size();
-            // per row and sublist: [begin, end) of each precision, as ket-table positions
-            std::vector<int> rkb[2], rke[2];
-            for (int prec = 0; prec < 2; prec++) {
-                rkb[prec].assign((size_t)nb * S, 0);
-                rke[prec].assign((size_t)nb * S, 0);
-            }
-            size_t first_combo = h->combos.size();
-            // row mode when the K rows of the bra shells fit in LDS
-            hfx_engine::RowCombo rc;
+            if (po.ncls[cb] == 0 || po.ncls[ck] == 0) continue;
+            Pending q{};
+            q.cb = cb;
+            q.ck = ck;
+            q.nb = po.ncls[cb];
+            q.S = (int)subs[ck].size();
+            hfx_engine::RowCombo& rc = q.rc;
             rc.cb = cb;
             rc.ck = ck;
             const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
             rc.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
             rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
+            if (rc.ncols >= 32768) return fail(h, "ket table: too many functions of one angular momentum");
             // the FP64 and FP32 launches slice differently: plan for the larger LDS of the two
             int nrow64 = 0, nab64 = 0, nrow32 = 0, nab32 = 0;
             hfx_row_shape(cb, ck, 0, &nrow64, &nab64);
@@ -666,58 +661,95 @@ int build_lists(hfx_t* h) {
             // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
             int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
             if (R == 0 && need <= h->lds_budget) R = 1;
-            const bool rowmode = h->opts.kernel_mode == 0 && R >= 1 && S <= HFX_SUB_MAX && rc.ncols < 32768;
-            for (int s = 0; s < S; s++) {
-                const std::vector<int>& kl = subl[ck][s];
-                const int npk = np_bucket(h->pairs[kl[0]].np);
-                const int soff = S <= HFX_SUB_MAX ? h->ktab[ck].t.off[s] : 0;
-                const int nk = (int)kl.size();
-                std::vector<double> kq(nk), cnp(nk + 1, 0.0);   // cnp: prefix sums of primitive-pair counts
-                std::vector<int> kpos(nk);
-                for (int j = 0; j < nk; j++) {
-                    kq[j] = h->pairs[kl[j]].Q;
-                    kpos[j] = pos[kl[j]];
-                    cnp[j + 1] = cnp[j] + h->pairs[kl[j]].np;
-                }
-                // number of kets with Q_i Q_j >= t: prefix of the descending list
-                auto prefix = [&](double qi, double t) -> int {
-                    if (qi <= 0) return 0;
-                    const double need = t / qi;
-                    return (int)(std::upper_bound(kq.begin(), kq.end(), need,
-                                                  [](double v, double e) { return v > e; }) - kq.begin());
-                };
-                std::vector<int> kmax(nb), k64(nb);
-                for (int i = 0; i < nb; i++) {
-                    const double qi = h->pairs[bl[i]].Q;
-                    int km = prefix(qi, thr);
-                    // same class: each unordered pair of pairs once, ket position <= bra position
-                    if (cb == ck) km = std::min(km, (int)(std::upper_bound(kpos.begin(), kpos.end(), i) - kpos.begin()));
-                    const int k6 = thr32 > 0 ? prefix(qi, thr32) : km;
-                    k64[i] = std::min(k6, km);
-                    kmax[i] = km;
-                }
-                hfx_engine::Combo cmb;
-                cmb.cb = cb;
-                cmb.ck = ck;
-                cmb.nket = nk;
-                cmb.npk = npk;
-                // the tile-mode lists (ket list, tiles, per-row ranges, gradient blocks)
-                // are built only for combinations that do not run in row mode
-                if (!rowmode && upload(h, &cmb.d_ket, kl, true)) return 1;
+            q.rowmode = h->opts.kernel_mode == 0 && R >= 1;
+            q.R = R;
+            q.dlds = dlds;
+            rc.R = R;
+            rc.dlds = dlds;
+            for (int prec = 0; prec < 2; prec++) {
+                if (dmalloc(h, (void**)&q.rkb[prec], (size_t)q.nb * q.S * sizeof(int), true)) return 1;
+                if (dmalloc(h, (void**)&q.rke[prec], (size_t)q.nb * q.S * sizeof(int), true)) return 1;
+                rc.d_kbeg[prec] = q.rkb[prec];
+                rc.d_kend[prec] = q.rke[prec];
+            }
+            q.stat = nstat;
+            nstat += (size_t)q.S * 4;
+            pend.push_back(q);
+        }
+    std::vector<double> stats(nstat), sums(pend.size() * 4, 0.0);
+    DeviceTemp d_stats, d_sums;
+    if (upload_temp(h, d_stats, stats) || upload_temp(h, d_sums, sums)) return 1;
+    for (Pending& q : pend) {
+        RangeArgs a{};
+        a.pairs = h->d_pairs;
+        a.bra_list = h->d_cls_list[q.cb];
+        a.kt = h->ktab[q.ck].t;
+        a.x = aux[q.ck];
+        a.nb = q.nb;
+        a.S = q.S;
+        a.same = q.cb == q.ck;
+        a.thr = thr;
+        a.thr32 = thr32;
+        for (int prec = 0; prec < 2; prec++) {
+            a.rkb[prec] = q.rkb[prec];
+            a.rke[prec] = q.rke[prec];
+        }
+        a.stats = (double*)d_stats.p + q.stat;
+        HIPCHK(h, hfx_launch_row_ranges(a, h->stream));
+    }
+    // summed spans and non-empty row groups of every row-mode launch (they choose its chunk)
+    for (size_t i = 0; i < pend.size(); i++) {
+        if (!pend[i].rowmode) continue;
+        for (int prec = 0; prec < 2; prec++) {
+            const BlockArgsL b{pend[i].rkb[prec], pend[i].rke[prec], pend[i].nb, pend[i].S, pend[i].R, 0};
+            HIPCHK(h, hfx_launch_group_spans(b, (double*)d_sums.p + 4 * i + 2 * prec, h->stream));
+        }
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(stats.data(), d_stats.p, nstat * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(sums.data(), d_sums.p, sums.size() * sizeof(double), hipMemcpyDeviceToHost));
+    bool any_tiles = false;
+    for (const Pending& q : pend) any_tiles = any_tiles || !q.rowmode;
+    if (any_tiles) {
+        hsub.resize(po.nsig);
+        if (po.nsig > 0)
+            HIPCHK(h, hipMemcpy(hsub.data(), po.sub_order, (size_t)po.nsig * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    for (size_t i = 0; i < pend.size(); i++) {
+        Pending& q = pend[i];
+        const int cb = q.cb, ck = q.ck, nb = q.nb, S = q.S;
+        std::vector<int> hkb[2], hke[2];
+        if (!q.rowmode)
+            for (int prec = 0; prec < 2; prec++) {
+                hkb[prec].resize((size_t)nb * S);
+                hke[prec].resize((size_t)nb * S);
+                HIPCHK(h, hipMemcpy(hkb[prec].data(), q.rkb[prec], hkb[prec].size() * sizeof(int), hipMemcpyDeviceToHost));
+                HIPCHK(h, hipMemcpy(hke[prec].data(), q.rke[prec], hke[prec].size() * sizeof(int), hipMemcpyDeviceToHost));
+            }
+        for (int s = 0; s < S; s++) {
+            hfx_engine::Combo cmb;
+            cmb.cb = cb;
+            cmb.ck = ck;
+            cmb.nket = subs[ck][s][1];
+            cmb.npk = subs[ck][s][0];
+            cmb.rowmode = q.rowmode;
+            const double* st = stats.data() + q.stat + 4 * (size_t)s;
+            cmb.nq[0] = st[0];
+            cmb.nprimq[0] = st[1];
+            cmb.nq[1] = st[2];
+            cmb.nprimq[1] = st[3];
+            if (q.rowmode) {
+                for (int prec = 0; prec < 2; prec++) cmb.ntiles[prec] = cmb.nq[prec] > 0 ? 1 : 0;   // statistics only
+            } else {
+                // tile mode: the sublist's ket list, per-row ranges relative to it, tiles, gradient blocks
+                const int soff = h->ktab[ck].t.off[s];
+                std::vector<int> kl(hsub.begin() + sub0[ck] + soff, hsub.begin() + sub0[ck] + soff + cmb.nket);
+                if (upload(h, &cmb.d_ket, kl, true)) return 1;
                 for (int prec = 0; prec < 2; prec++) {
                     std::vector<int> kb(nb), ke(nb), ti0, tj0;
-                    for (int i = 0; i < nb; i++) {
-                        kb[i] = prec == 0 ? 0 : k64[i];
-                        ke[i] = prec == 0 ? k64[i] : kmax[i];
-                        const double nqi = std::max(0, ke[i] - kb[i]);
-                        cmb.nq[prec] += nqi;
-                        if (ke[i] > kb[i]) cmb.nprimq[prec] += h->pairs[bl[i]].np * (cnp[ke[i]] - cnp[kb[i]]);
-                        rkb[prec][(size_t)i * S + s] = soff + kb[i];
-                        rke[prec][(size_t)i * S + s] = soff + std::max(kb[i], ke[i]);
-                    }
-                    if (rowmode) {
-                        cmb.ntiles[prec] = cmb.nq[prec] > 0 ? 1 : 0;   // marks quartets for the statistics
-                        continue;
+                    for (int r = 0; r < nb; r++) {
+                        kb[r] = hkb[prec][(size_t)r * S + s] - soff;
+                        ke[r] = hke[prec][(size_t)r * S + s] - soff;
                     }
                     for (int r0 = 0; r0 < nb; r0 += HFX_TILE_ROWS) {
                         int lo = 1 << 30, hi = 0;
@@ -739,20 +771,20 @@ int build_lists(hfx_t* h) {
                         if (upload(h, &cmb.d_kbeg[prec], kb, true)) return 1;
                         if (upload(h, &cmb.d_kend[prec], ke, true)) return 1;
                     }
-                    // one-row blocks of the gradient kernel
+                    // one-row blocks of the fallback gradient kernel
                     std::vector<int> br, bk;
                     double tot = 0;
                     int ngroups = 0;
-                    for (int i = 0; i < nb; i++)
-                        if (ke[i] > kb[i]) {
-                            tot += ke[i] - kb[i];
+                    for (int r = 0; r < nb; r++)
+                        if (ke[r] > kb[r]) {
+                            tot += ke[r] - kb[r];
                             ngroups++;
                         }
                     if (ngroups == 0) continue;
                     const int chunk = block_chunk(tot, ngroups);
-                    for (int i = 0; i < nb; i++)
-                        for (int k0 = kb[i]; k0 < ke[i]; k0 += chunk) {
-                            br.push_back(i);
+                    for (int r = 0; r < nb; r++)
+                        for (int k0 = kb[r]; k0 < ke[r]; k0 += chunk) {
+                            br.push_back(r);
                             bk.push_back(k0);
                         }
                     cmb.gchunk[prec] = chunk;
@@ -760,101 +792,32 @@ int build_lists(hfx_t* h) {
                     if (upload(h, &cmb.d_gblk_row[prec], br, true)) return 1;
                     if (upload(h, &cmb.d_gblk_k0[prec], bk, true)) return 1;
                 }
-                if (cmb.ntiles[0] + cmb.ntiles[1] > 0) h->combos.push_back(cmb);
             }
-            for (size_t ci = first_combo; ci < h->combos.size(); ci++) h->combos[ci].rowmode = rowmode;
-            if (!rowmode) continue;
-            rc.R = R;
-            rc.dlds = dlds;
-            {
-                const FlopModel fm = flop_model(pair_class_la(cb), pair_class_lb(cb), lc, ld);
-                const double nint = (double)ncart(pair_class_la(cb)) * ncart(pair_class_lb(cb)) * ncart(lc) * ncart(ld);
-                for (size_t ci = first_combo; ci < h->combos.size(); ci++)
-                    for (int prec = 0; prec < 2; prec++)
-                        rc.cost[prec] += h->combos[ci].nprimq[prec] * fm.per_prim + h->combos[ci].nq[prec] * nint * fm.per_int;
-            }
-            // blocks: (first row, first ket position) for groups of R rows, chunks of the
-            // table positions that meet at least one of the group's ranges
-            for (int prec = 0; prec < 2; prec++) {
-                double tot = 0;
-                int ngroups = 0;
-                for (int g0 = 0; g0 < nb; g0 += R) {
-                    double span = 0;
-                    for (int s = 0; s < S; s++) {
-                        int lo = 1 << 30, hi = 0;
-                        for (int i = g0; i < std::min(nb, g0 + R); i++) {
-                            const size_t x = (size_t)i * S + s;
-                            if (rke[prec][x] > rkb[prec][x]) {
-                                lo = std::min(lo, rkb[prec][x]);
-                                hi = std::max(hi, rke[prec][x]);
-                            }
-                        }
-                        if (hi > lo) span += hi - lo;
-                    }
-                    if (span > 0) {
-                        tot += span;
-                        ngroups++;
-                    }
-                }
-                if (ngroups == 0) continue;
-                const int chunk = block_chunk(tot, ngroups, h->row_blocks);
-                std::vector<int> br, bk;
-                std::vector<std::pair<int, int>> rng;
-                for (int g0 = 0; g0 < nb; g0 += R) {
-                    rng.clear();
-                    for (int s = 0; s < S; s++) {
-                        int lo = 1 << 30, hi = 0;
-                        for (int i = g0; i < std::min(nb, g0 + R); i++) {
-                            const size_t x = (size_t)i * S + s;
-                            if (rke[prec][x] > rkb[prec][x]) {
-                                lo = std::min(lo, rkb[prec][x]);
-                                hi = std::max(hi, rke[prec][x]);
-                            }
-                        }
-                        if (hi > lo) rng.push_back({lo, hi});
-                    }
-                    if (rng.empty()) continue;
-                    for (int k0 = rng.front().first; k0 < rng.back().second; k0 += chunk) {
-                        bool meets = false;
-                        for (auto& q : rng) meets = meets || (q.first < k0 + chunk && q.second > k0);
-                        if (!meets) continue;
-                        br.push_back(g0);
-                        bk.push_back(k0);
-                    }
-                }
-                if (h->opts.verbose) {
-                    // lane use of the launch: primitive quartets over the lane slots of the
-                    // wavefronts that hold at least one ket
-                    double used = 0, slots = 0;
-                    for (size_t b = 0; b < br.size(); b++)
-                        for (int s = 0; s < S; s++) {
-                            int lo = 1 << 30, hi = 0;
-                            for (int i = br[b]; i < std::min(nb, br[b] + R); i++) {
-                                const size_t x = (size_t)i * S + s;
-                                if (rke[prec][x] > rkb[prec][x]) {
-                                    lo = std::min(lo, rkb[prec][x]);
-                                    hi = std::max(hi, rke[prec][x]);
-                                }
-                            }
-                            const int j0 = std::max(bk[b], lo), j1 = std::min(bk[b] + chunk, hi);
-                            if (j1 <= j0) continue;
-                            const double w = (double)h->ktab[ck].t.np[s] * h->pairs[bl[br[b]]].np;
-                            used += w * (j1 - j0);
-                            slots += w * ((j1 - j0 + 63) / 64) * 64;
-                        }
-                    fprintf(stderr, "hfx:   rows (%d%d|%d%d) %s: lane use %.3f of %.4g primitive-quartet slots\n",
-                            pair_class_la(cb), pair_class_lb(cb), lc, ld, prec ? "fp32" : "fp64",
-                            slots > 0 ? used / slots : 0.0, slots);
-                }
-                rc.chunk[prec] = chunk;
-                rc.nblk[prec] = (int)br.size();
-                if (upload(h, &rc.d_kbeg[prec], rkb[prec], true)) return 1;
-                if (upload(h, &rc.d_kend[prec], rke[prec], true)) return 1;
-                if (upload(h, &rc.d_blk_row[prec], br, true)) return 1;
-                if (upload(h, &rc.d_blk_k0[prec], bk, true)) return 1;
-            }
-            h->rcombos.push_back(rc);
+            if (cmb.ntiles[0] + cmb.ntiles[1] > 0) h->combos.push_back(cmb);
         }
+        if (!q.rowmode) continue;
+        // row mode: modelled cost (orders the launches), then the blocks of each precision on the GPU
+        hfx_engine::RowCombo& rc = q.rc;
+        const FlopModel fm = flop_model(pair_class_la(cb), pair_class_lb(cb), pair_class_la(ck), pair_class_lb(ck));
+        const double nint = (double)ncart(pair_class_la(cb)) * ncart(pair_class_lb(cb)) * ncart(pair_class_la(ck)) *
+                            ncart(pair_class_lb(ck));
+        for (int s = 0; s < S; s++)
+            for (int prec = 0; prec < 2; prec++) {
+                const double* st = stats.data() + q.stat + 4 * (size_t)s + 2 * prec;
+                rc.cost[prec] += st[1] * fm.per_prim + st[0] * nint * fm.per_int;
+            }
+        for (int prec = 0; prec < 2; prec++) {
+            const double tot = sums[4 * i + 2 * prec], ngroups = sums[4 * i + 2 * prec + 1];
+            if (ngroups == 0) continue;
+            const int chunk = block_chunk(tot, (int)ngroups, h->row_blocks);
+            const BlockArgsL b{q.rkb[prec], q.rke[prec], nb, S, q.R, chunk};
+            HIPCHK(h, hfx_group_blocks(b, &rc.d_blk_row[prec], &rc.d_blk_k0[prec], &rc.nblk[prec], h->geom_allocs,
+                                       h->stream));
+            rc.chunk[prec] = chunk;
+        }
+        h->rcombos.push_back(rc);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     if (h->opts.verbose) {
         double q64 = 0, q32 = 0;
         for (auto& c : h->combos) {

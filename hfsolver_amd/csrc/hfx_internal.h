@@ -2,6 +2,9 @@
 // (hfx_kernels.hip). Not part of the public ABI.
 #pragma once
 
+#include <array>
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 #include "hfx_common.h"
@@ -61,6 +64,64 @@ struct KetTab {
     long long psoff[HFX_SUB_MAX];
     long long psoff_f[HFX_SUB_MAX];
 };
+
+// ---- quartet lists built on the GPU (hfx_lists.hip) ----
+// Significant pairs in class order (Q-descending within a class) and in
+// (class, primitive-count bucket) order; runs = (class, bucket, count) of the
+// second order; pos = position of every significant pair in its class list.
+struct PairOrder {
+    int* cls_order = nullptr;   // [npairs] (device, allocated by the caller)
+    int* sub_order = nullptr;   // [npairs]
+    int* pos = nullptr;         // [npairs], -1 for pairs that are not significant
+    int ncls[8] = {0};
+    int nsig = 0;
+    std::vector<std::array<int, 3>> runs;
+};
+// Mutable views of one class's ket table for the fill kernel.
+struct KetOut {
+    KetInfo* info;
+    double* cd;
+    int* shcd;
+    float* qk;
+    double* prim;
+    float* primf;
+};
+// Per ket of one class table: bound, position in the class list, primitive
+// count ([nk + 1], the last 0), and the exclusive prefix sums of the counts
+// over the table ([nk + 1]).
+struct KetAux {
+    double* q;
+    int* pos;
+    int* np;
+    int* cnp;
+};
+// Ket ranges of one class combination (bra rows = the bra class list).
+struct RangeArgs {
+    const ShellPair* pairs;
+    const int* bra_list;
+    KetTab kt;
+    KetAux x;
+    int nb, S;
+    int same;            // bra class == ket class: kets up to the diagonal
+    double thr, thr32;
+    int* rkb[2];         // [nb * S] per precision
+    int* rke[2];
+    double* stats;       // [S][nq64, nprimq64, nq32, nprimq32]
+};
+// Row-mode blocks of one class combination and precision.
+struct BlockArgsL {
+    const int* rkb;
+    const int* rke;
+    int nb, S, R, chunk;
+};
+hipError_t hfx_pair_order(const ShellPair* d_pairs, int np, double thr, PairOrder& o, hipStream_t s);
+hipError_t hfx_launch_ket_fill(const KetTab& t, const KetOut& o, const KetAux& x, const int* sub_order,
+                               const ShellPair* pairs, const PrimPair* prims, const PrimExp* pexp, const int* pos,
+                               hipStream_t s);
+hipError_t hfx_launch_row_ranges(const RangeArgs& a, hipStream_t s);
+hipError_t hfx_launch_group_spans(const BlockArgsL& a, double* sums, hipStream_t s);
+hipError_t hfx_group_blocks(const BlockArgsL& a, int** brow, int** bk0, int* nblk, std::vector<void*>& allocs,
+                            hipStream_t s);
 
 // Arguments of one row-mode J/K launch (bra class X, ket class Y, X >= Y):
 // block b owns bra rows blk_row[b] .. blk_row[b] + R - 1 and the ket

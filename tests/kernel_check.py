@@ -39,6 +39,7 @@ def build_host_sanitized():
     code only, sanitizer runtimes linked statically), linked with the gfx950
     kernel objects of the library build (hfsolver_amd/_build). Runs on a GPU."""
     kobj = [os.path.join(ROOT, "hfsolver_amd", "_build", "hfx_kernels_%d.o" % p) for p in range(1, 8)]
+    kobj.append(os.path.join(ROOT, "hfsolver_amd", "_build", "hfx_lists.o"))
     missing = [o for o in kobj if not os.path.exists(o)]
     if missing:
         raise RuntimeError("kernel objects missing (python -m hfsolver_amd.build first): %s" % missing[0])
