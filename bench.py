@@ -10,7 +10,8 @@ E_tot(FP64 SCF)|, the first-order |dE| = |1/2 tr P (G_mixed - G_fp64)| and
 max |dF_ij| at the density, the rate with host buffers (PCIe included), the
 analytic gradient (time and roofline of the derivative kernels), the
 mixed-precision force workload (10 warm-started energy + force evaluations on
-perturbed geometries, against the FP64 engine), and the CPU baselines: the
+perturbed geometries, against the FP64 engine; the SCF densities by canonical
+purification, one eigendecomposition per SCF), and the CPU baselines: the
 oracle (THO restatement of the reference's integrals) on a sample of the same
 quartets and on one full 16-water build, and the compiled reference's own
 Fortran path on 1 and 2 waters (one core; it cannot run 16 or 64 waters).
@@ -208,8 +209,9 @@ def main():
                          "mixed-precision force workload); reported under 'forces'")
     ap.add_argument("--incremental", type=float, default=0.0, metavar="THR",
                     help="force workload: incremental Fock builds with density-weighted screening at THR (0: off)")
-    ap.add_argument("--purify", action="store_true",
-                    help="force workload: SCF densities by canonical purification instead of eigenvectors")
+    ap.add_argument("--eigenvectors", action="store_true",
+                    help="force workload: SCF densities from an eigendecomposition every iteration instead of "
+                         "canonical purification (matrix products; one eigendecomposition at convergence)")
     ap.add_argument("--verbose", action="store_true", help="per-class screening statistics on stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (known bytes) to scale HBM counters")
@@ -353,7 +355,7 @@ def main():
         from hfsolver_amd.grad import ForceField
         rng = np.random.Generator(np.random.PCG64(args.seed + 1))
         kw = dict(Nscf=80, tolE=1e-10, tolP=1e-7, diis=True, eig="device", incremental=args.incremental,
-                  purify=args.purify)
+                  purify=not args.eigenvectors)
         ffm = ForceField(atno, xyz, fp32_threshold=args.fp32_threshold, **kw)
         ff64 = ForceField(atno, xyz, fp32_threshold=0.0, **kw)
         if r64 is not None:   # the first evaluation starts from the base geometry's density
@@ -382,7 +384,7 @@ def main():
                   "force_error_rms_vs_fp64": float(np.sqrt(np.mean(np.square(grms)))),
                   "abs_dE_max_vs_fp64": max(dEs),
                   "perturbation": "uniform +-0.01 A per coordinate about the base geometry, seed %d" % (args.seed + 1),
-                  "incremental_threshold": args.incremental, "purification": bool(args.purify)}
+                  "incremental_threshold": args.incremental, "purification": not args.eigenvectors}
 
     cpu = None
     if not args.no_cpu_baseline:
