@@ -304,6 +304,9 @@ def main():
     log("  mixed: %.4f s / build (median %.4f)" % (tmx / args.steps, np.median(tsmx)))
     G64 = dG64.cpu().numpy().reshape(n, n)
     Gmx = dGmx.cpu().numpy().reshape(n, n)
+    # algorithm-neutral lower bound of a build: sum_q K_q 3 nroots(L_q) N_q (the Rys assembly alone)
+    lower64 = sum(e64.work_model()[1])
+    lowermx = sum(emx.work_model()[1])
     dE = 0.5 * float(np.sum(P * (Gmx - G64)))
     dF = float(np.abs(Gmx - G64).max())
     E64 = 0.5 * float(np.sum(P * (2 * H + G64)))
@@ -455,8 +458,10 @@ def main():
                      "kernel": "jk_row_kernel (all class launches of one mixed-precision build)",
                      "jk_seconds_mixed": float(jkmx), "flops_mixed": float(fmx64 + fmx32),
                      "flops_mixed_fp64_part": float(fmx64), "flops_mixed_fp32_part": float(fmx32),
+                     "lower_bound_flops": lowermx, "lower_bound_frac": lowermx / jkmx / PEAK_VALU,
                      "fp64_build": {"achieved": roof64 / 1e12, "frac": roof64 / PEAK_FP64,
-                                    "jk_seconds": float(jk64), "flops": float(f64), "traffic": traffic64},
+                                    "jk_seconds": float(jk64), "flops": float(f64), "traffic": traffic64,
+                                    "lower_bound_flops": lower64, "lower_bound_frac": lower64 / jk64 / PEAK_FP64},
                      "traffic_source": TRAFFIC_FILE if (traffic or traffic64) else None},
         "cpu_baseline": cpu,
     }

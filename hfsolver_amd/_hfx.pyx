@@ -232,3 +232,11 @@ cdef class Engine:
         cdef double a = 0, b = 0, c = 0
         self._check(c_hfx.hfx_screening(h, &a, &b, &c))
         return a, b, c
+
+    def work_model(self):
+        """((model FP64, model FP32), (lower bound FP64, lower bound FP32)) FLOPs of one Fock build."""
+        cdef c_hfx.hfx_t *h = self._live()
+        cdef double m[2]
+        cdef double lo[2]
+        self._check(c_hfx.hfx_work_model(h, m, lo))
+        return (m[0], m[1]), (lo[0], lo[1])

@@ -44,5 +44,6 @@ cdef extern from "hfx.h":
     int hfx_pair_bounds(const hfx_t *h, int *npairs, int *A, int *B, double *Q)
     int hfx_set_density_screening(hfx_t *h, double threshold)
     int hfx_screening(const hfx_t *h, double *npairs, double *nq_fp64, double *nq_fp32)
+    int hfx_work_model(const hfx_t *h, double *model, double *lower)
     const char *hfx_last_error(const hfx_t *h)
     void hfx_destroy(hfx_t *h)

@@ -1771,6 +1771,27 @@ int hfx_screening(const hfx_t* h, double* npairs, double* nq64, double* nq32) {
     return 0;
 }
 
+int hfx_work_model(const hfx_t* h, double* model, double* lower) {
+    if (!h) return 1;
+    double m[2] = {0, 0}, lo[2] = {0, 0};
+    for (const auto& c : h->combos) {
+        const int la = pair_class_la(c.cb), lb = pair_class_lb(c.cb), lc = pair_class_la(c.ck),
+                  ld = pair_class_lb(c.ck);
+        const FlopModel fm = flop_model(la, lb, lc, ld);
+        const double nint = (double)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+        const int nroots = (la + lb + lc + ld) / 2 + 1;
+        for (int prec = 0; prec < 2; prec++) {
+            m[prec] += c.nprimq[prec] * fm.per_prim + c.nq[prec] * nint * fm.per_int;
+            lo[prec] += c.nprimq[prec] * 3.0 * nroots * nint;
+        }
+    }
+    for (int prec = 0; prec < 2; prec++) {
+        if (model) model[prec] = m[prec];
+        if (lower) lower[prec] = lo[prec];
+    }
+    return 0;
+}
+
 const char* hfx_last_error(const hfx_t* h) { return h ? h->err.c_str() : "null handle"; }
 
 void hfx_destroy(hfx_t* h) {

@@ -54,6 +54,7 @@ SIGNATURES = {
     "hfx_screening": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "hfx_set_density_screening": (_I, [_P, _D]),
     "hfx_pair_bounds": (_I, [_P, ctypes.POINTER(_I), _P, _P, _P]),
+    "hfx_work_model": (_I, [_P, _P, _P]),
     "hfx_last_error": (ctypes.c_char_p, [_P]),
     "hfx_destroy": (None, [_P]),
 }
@@ -241,6 +242,14 @@ class Engine:
         Q = np.zeros(npr.value)
         self._check(lib().hfx_pair_bounds(self._h, ctypes.byref(npr), _ptr(A), _ptr(B), _ptr(Q)))
         return A, B, Q
+
+    def work_model(self):
+        """((model FP64, model FP32), (lower bound FP64, lower bound FP32)) FLOPs of one Fock build:
+        the frozen per-class model and the Rys-assembly lower bound sum_q K_q 3 nroots N_q."""
+        m = np.zeros(2)
+        lo = np.zeros(2)
+        self._check(lib().hfx_work_model(self._h, _ptr(m), _ptr(lo)))
+        return (float(m[0]), float(m[1])), (float(lo[0]), float(lo[1]))
 
     def set_density_screening(self, threshold):
         """Skip quartets whose Schwarz bound times the largest density factor is

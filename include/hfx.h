@@ -201,6 +201,15 @@ int hfx_set_density_screening(hfx_t *h, double threshold);
  * screened shell quartets in FP64 / FP32. */
 int hfx_screening(const hfx_t *h, double *npairs, double *nq_fp64, double *nq_fp32);
 
+/* Work of one Fock build at the current set-up (no counterpart in the
+ * reference): model[0], model[1] = modelled FLOPs of the FP64 / FP32 launches
+ * (the frozen per-class operation counts hfx_stats reports); lower[0],
+ * lower[1] = the algorithm-neutral lower bound, the sum over screened quartets
+ * of K_q * 3 nroots(L_q) * N_q (primitive quartets K_q times the Rys
+ * assembly of each of the N_q integrals: nroots = floor(L_q / 2) + 1 roots,
+ * a product and a multiply-add each). Either array may be NULL. */
+int hfx_work_model(const hfx_t *h, double *model, double *lower);
+
 const char *hfx_last_error(const hfx_t *h);
 void hfx_destroy(hfx_t *h);
 

@@ -60,3 +60,29 @@ def test_quartet_lists_match_a_direct_count(thr, thr32, kernel_mode):
     nsig, n64, n32 = direct_counts(A, B, Q, L, thr, thr32)
     assert (int(npairs), int(nq64), int(nq32)) == (nsig, n64, n32)
     assert n32 > 0 and n64 > 0
+
+
+def test_work_model_matches_the_build_statistics():
+    """hfx_work_model: its model FLOPs are those a Fock build reports, and the
+    Rys-assembly lower bound (sum_q K_q 3 nroots N_q) lies below them; the
+    Cython binding returns the same figures."""
+    from hfsolver_amd import hfx
+    atno, xyz = water_cluster(4, seed=4)
+    b = Basis(atno, xyz)
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((b.nbf, 10))
+    P = X @ X.T / 10
+    eng = Engine(b, schwarz_threshold=1e-12, fp32_threshold=1e-5)
+    try:
+        (m64, m32), (l64, l32) = eng.work_model()
+        st = eng.fock_2e(P)[3]
+    finally:
+        eng.close()
+    assert (m64, m32) == pytest.approx((st.flops_fp64, st.flops_fp32), rel=1e-12)
+    assert 0 < l64 < m64 and 0 < l32 < m32
+    ce = hfx.Engine(b, schwarz_threshold=1e-12, fp32_threshold=1e-5)
+    try:
+        (c64, c32), (cl64, cl32) = ce.work_model()
+        assert (c64, c32, cl64, cl32) == (m64, m32, l64, l32)
+    finally:
+        ce.close()
