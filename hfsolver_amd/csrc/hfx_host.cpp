@@ -1071,7 +1071,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.offd = c.offd;
             r.bf_c = h->d_bfL[pair_class_la(c.ck)];
             r.bf_d = h->d_bfL[pair_class_lb(c.ck)];
-            r.dlds = c.dlds;
+            r.dlds = c.dlds && hfx_row_stage_density(c.cb, c.ck, prec == 1);
             r.D = h->d_D;
             r.J = h->d_Jb;
             r.K = h->d_Kb;

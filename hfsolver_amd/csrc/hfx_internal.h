@@ -294,6 +294,17 @@ HFX_HD constexpr int hfx_row_threads(int X, int Y, bool fp32) {
     if ((X == 2 && Y == 1) || (X == 3 && Y == 3) || (X == 5 && Y == 3)) return 512;   // (11|10) (20|20) (22|20)
     return 256;
 }
+// Whether a row-mode J/K launch of class (X|Y) stages the density rows of its
+// bra shells in LDS (when they fit the LDS budget). Not for these classes,
+// whose launches measured faster reading the density through the caches
+// (64 waters, launches serialised: FP64 (10|10) -1 % in the FP64 build and
+// -16 % in the mixed build, (11|11) -4 %, (21|21) -10 %, (22|11) -1 %,
+// (22|22) -9 %; FP32 (22|22) -12 %); every other class is faster staging them.
+HFX_HD constexpr bool hfx_row_stage_density(int X, int Y, bool fp32) {
+    if (fp32) return !(X == 5 && Y == 5);
+    return !((X == 1 && Y == 1) || (X == 2 && Y == 2) || (X == 4 && Y == 4) || (X == 5 && Y == 2) ||
+             (X == 5 && Y == 5));
+}
 // Threads of a gradient row-mode block of class (X|Y) in FP32 or FP64: 256,
 // or 512 for the FP32 classes (21|11), (21|10), (20|11) (10-24 % faster per
 // class at 64 waters; every other class is slower with 512 or 1024).
