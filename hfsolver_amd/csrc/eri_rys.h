@@ -52,11 +52,17 @@ struct QClass {
 #define HFX_JK_SPLIT_F32 162
 #endif
 HFX_HD constexpr int jk_split_limit(bool fp32) { return fp32 ? HFX_JK_SPLIT_F32 : HFX_JK_SPLIT_F64; }
+// Classes sliced over their c-components alone although they fit the limit:
+// FP64 (22|10) (108 integrals per thread spill 744 bytes; three slices of 36).
+HFX_HD constexpr bool split_c_only(int la, int lb, int lc, int ld, bool fp32) {
+    return !fp32 && la == 2 && lb == 2 && lc == 1 && ld == 0;
+}
 HFX_HD constexpr bool split_a(int la, int lb, int lc, int ld, bool fp32 = false) {
-    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > jk_split_limit(fp32);
+    return !split_c_only(la, lb, lc, ld, fp32) && ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > jk_split_limit(fp32);
 }
 HFX_HD constexpr bool split_c(int la, int lb, int lc, int ld, bool fp32 = false) {
-    return split_a(la, lb, lc, ld, fp32) && ncart(lb) * ncart(lc) * ncart(ld) > jk_split_limit(fp32);
+    return split_c_only(la, lb, lc, ld, fp32) ||
+           (split_a(la, lb, lc, ld, fp32) && ncart(lb) * ncart(lc) * ncart(ld) > jk_split_limit(fp32));
 }
 HFX_HD constexpr int nsplit(int la, int lb, int lc, int ld, bool fp32 = false) {
     return (split_a(la, lb, lc, ld, fp32) ? ncart(la) : 1) * (split_c(la, lb, lc, ld, fp32) ? ncart(lc) : 1);

@@ -287,11 +287,12 @@ HFX_HD constexpr int hfx_row_threads(int X, int Y, bool fp32) {
     if (fp32) {
         if (X == 3 && Y == 1) return 1024;                            // (20|10)
         if ((X == 1 && Y == 1) || (X == 2 && Y == 1) || (X == 4 && Y == 1) || (X == 4 && Y == 3) ||
-            (X == 5 && Y == 4))
-            return 512;                                               // (10|10) (11|10) (21|10) (21|20) (22|21)
+            (X == 5 && Y == 4) || (X == 5 && Y == 0))
+            return 512;                                    // (10|10) (11|10) (21|10) (21|20) (22|21) (22|00)
         return 256;
     }
-    if ((X == 2 && Y == 1) || (X == 3 && Y == 3) || (X == 5 && Y == 3)) return 512;   // (11|10) (20|20) (22|20)
+    if ((X == 2 && Y == 1) || (X == 3 && Y == 3) || (X == 5 && Y == 3) || (X == 3 && Y == 2))
+        return 512;                                        // (11|10) (20|20) (22|20) (20|11)
     return 256;
 }
 // Whether a row-mode J/K launch of class (X|Y) stages the density rows of its
