@@ -57,6 +57,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class JKTimedEngine:
+    """An Engine whose device-resident Fock builds keep their J/K seconds
+    (hfx_stats.jk_seconds, HIP events) in `jk`: the per-SCF-iteration J+K time."""
+
+    def __init__(self, eng):
+        self.eng, self.jk = eng, []
+
+    def __getattr__(self, name):
+        return getattr(self.eng, name)
+
+    def fock_2e_device(self, *a, **kw):
+        st = self.eng.fock_2e_device(*a, **kw)
+        self.jk.append(st.jk_seconds)
+        return st
+
+
 def cpu_baseline(eng, basis, nq_total, target_s=12.0, seed=0):
     """The oracle (THO restatement of the reference's cints.c, all host threads)
     timed on a uniform random sample of this build's screened shell quartets;
@@ -268,10 +284,13 @@ def main():
         kw = dict(Nscf=80, tolE=1e-9, tolP=1e-6, diis=True)
         for name, eng in (("fp64", e64), ("mixed", emx)):
             t = time.time()
-            r = doscf(H, S, nocc, lambda X, e=eng: e.fock_2e(X)[0], eig="device", engine=eng, **kw)
-            scf[name] = {"Etot": r["E0"] + Enuc, "iterations": r["iterations"], "seconds": time.time() - t}
-            log("  %s SCF: E_tot %.10f Eh in %d iterations (%.1f s)" % (name, scf[name]["Etot"], r["iterations"],
-                                                                     time.time() - t))
+            te = JKTimedEngine(eng)
+            r = doscf(H, S, nocc, lambda X, e=eng: e.fock_2e(X)[0], eig="device", engine=te, **kw)
+            scf[name] = {"Etot": r["E0"] + Enuc, "iterations": r["iterations"], "seconds": time.time() - t,
+                         "jk_seconds_per_iteration_median": float(np.median(te.jk)),
+                         "jk_seconds_per_iteration_max": float(np.max(te.jk)), "fock_builds": len(te.jk)}
+            log("  %s SCF: E_tot %.10f Eh in %d iterations (%.1f s); J+K per iteration median %.4f s, max %.4f s"
+                % (name, scf[name]["Etot"], r["iterations"], time.time() - t, np.median(te.jk), np.max(te.jk)))
             if name == "fp64":
                 r64 = r
         P = r64["P"]
