@@ -77,7 +77,9 @@ struct hfx_engine {
     std::vector<int> bf_atom;
     std::vector<int> bfL[HFX_LMAX + 1];   // functions of the shells of each L, in shell order
     int* d_bfL[HFX_LMAX + 1] = {nullptr};
-    size_t lds_budget = 120 * 1024;   // row-mode LDS per block (the heaviest classes run one block per CU)
+    // row-mode LDS per block (a class whose K rows need more runs as tiles; the
+    // heaviest classes run one block per CU)
+    size_t lds_budget = HFX_ROW_LDS_MAX;
     size_t dlds_budget = 120 * 1024;  // largest row-mode LDS per block with the density rows staged
     // pairs
     std::vector<ShellPair> pairs;

@@ -313,7 +313,9 @@ HFX_HD constexpr int hfx_grad_threads(int X, int Y, bool fp32) {
     return fp32 && ((X == 4 && Y == 2) || (X == 4 && Y == 1) || (X == 3 && Y == 2)) ? 512 : 256;
 }
 static constexpr int HFX_AUX_STREAMS = 3; // extra streams of the class launches (at most)
-static constexpr size_t HFX_ROW_LDS_MAX = 128 * 1024;   // dynamic LDS of a row-mode block
+// dynamic LDS of a row-mode block: gfx950 gives a workgroup up to 160 KB, of
+// which 4 KB stay for the kernels' static arrays
+static constexpr size_t HFX_ROW_LDS_MAX = 156 * 1024;
 static constexpr int HFX_TILE_ROWS = 4;   // one bra row per wavefront
 static constexpr int HFX_TILE_COLS = 64;
 
