@@ -67,6 +67,24 @@ HFX_HD constexpr bool split_c(int la, int lb, int lc, int ld, bool fp32 = false)
 HFX_HD constexpr int nsplit(int la, int lb, int lc, int ld, bool fp32 = false) {
     return (split_a(la, lb, lc, ld, fp32) ? ncart(la) : 1) * (split_c(la, lb, lc, ld, fp32) ? ncart(lc) : 1);
 }
+// Narrow row launches, for molecules whose K rows outgrow the LDS of a block:
+// a class with several a-components that the policy above leaves unsliced over
+// a is sliced over them (then over c by the same limit), so that a row block
+// holds one K row of the a-shell instead of ncart(la).
+HFX_HD constexpr bool narrow_differs(int la, int lb, int lc, int ld, bool fp32) {
+    return ncart(la) > 1 && !split_a(la, lb, lc, ld, fp32);
+}
+HFX_HD constexpr bool split_a_rows(int la, int lb, int lc, int ld, bool fp32, bool narrow) {
+    return (narrow && narrow_differs(la, lb, lc, ld, fp32)) || split_a(la, lb, lc, ld, fp32);
+}
+HFX_HD constexpr bool split_c_rows(int la, int lb, int lc, int ld, bool fp32, bool narrow) {
+    return narrow && narrow_differs(la, lb, lc, ld, fp32) ? ncart(lb) * ncart(lc) * ncart(ld) > jk_split_limit(fp32)
+                                                          : split_c(la, lb, lc, ld, fp32);
+}
+HFX_HD constexpr int nsplit_rows(int la, int lb, int lc, int ld, bool fp32, bool narrow) {
+    return (split_a_rows(la, lb, lc, ld, fp32, narrow) ? ncart(la) : 1) *
+           (split_c_rows(la, lb, lc, ld, fp32, narrow) ? ncart(lc) : 1);
+}
 
 // One Cartesian direction of one Rys root: 2-D integrals, bra and ket
 // transfers, selection of the a / c exponent (ea, ec) when slicing.

@@ -116,6 +116,7 @@ struct hfx_engine {
     struct RowCombo {
         int cb, ck;
         int R = 1, dlds = 0, ncols = 0, offd = 0;
+        bool narrow = false;   // launches sliced by split_a_rows / split_c_rows (K rows too wide otherwise)
         int nblk[2] = {0, 0};
         int chunk[2] = {0, 0};
         double cost[2] = {0, 0};   // modelled flops of the launch (orders the launches)
@@ -136,6 +137,7 @@ struct hfx_engine {
     int* d_dsh_f0 = nullptr;
     int* d_dsh_nc = nullptr;
     int row_dlds = 1;
+    int row_narrow = 1;   // narrow row launches where the K rows do not fit (HFX_ROW_NARROW)
     double npairs_sig = 0;
     // device memory
     ShellPair* d_pairs = nullptr;
@@ -663,6 +665,17 @@ int build_lists(hfx_t* h) {
             // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
             int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
             if (R == 0 && need <= h->lds_budget) R = 1;
+            if (R == 0 && h->row_narrow) {
+                // narrow launches: one K row of the a-shell per bra, one row per block
+                hfx_row_shape(cb, ck, 4, &nrow64, &nab64);
+                hfx_row_shape(cb, ck, 5, &nrow32, &nab32);
+                dlds = h->row_dlds;
+                if (dlds && row_lds(1, 1) > std::min(h->lds_budget, h->dlds_budget)) dlds = 0;
+                if (row_lds(1, dlds) <= h->lds_budget) {
+                    R = 1;
+                    rc.narrow = true;
+                }
+            }
             q.rowmode = h->opts.kernel_mode == 0 && R >= 1;
             q.R = R;
             q.dlds = dlds;
@@ -833,9 +846,9 @@ int build_lists(hfx_t* h) {
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck),
                     c.npk, c.rowmode ? "rows" : "tiles", c.nq[0], c.nq[1], c.nprimq[0], c.nprimq[1]);
         for (auto& c : h->rcombos)
-            fprintf(stderr, "hfx:   rows (%d%d|%d%d): R=%d dlds=%d ncols=%d blocks %d / %d chunk %d / %d\n",
+            fprintf(stderr, "hfx:   rows (%d%d|%d%d): R=%d dlds=%d narrow=%d ncols=%d blocks %d / %d chunk %d / %d\n",
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck), c.R, c.dlds,
-                    c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
+                    (int)c.narrow, c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
     }
     return 0;
 }
@@ -972,6 +985,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (const char* e = std::getenv("HFX_ROW_RMAX")) h->row_rmax = std::max(1, std::min(HFX_ROWS_MAX, std::atoi(e)));
     if (const char* e = std::getenv("HFX_ROW_PAIR_KB")) h->row_pair_bytes = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
+    if (const char* e = std::getenv("HFX_ROW_NARROW")) h->row_narrow = std::atoi(e) != 0;
     if (const char* e = std::getenv("HFX_ROW_DLDS_KB")) h->dlds_budget = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_BLOCKS")) h->row_blocks = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("HFX_STREAMS")) h->nstreams = std::max(1, std::min(HFX_AUX_STREAMS + 1, std::atoi(e)));
@@ -1081,7 +1095,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.dsh = h->dthr > 0 ? h->d_dsh : nullptr;
             r.dthr = (float)h->dthr;
             r.nsh = nsh;
-            HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], s));
+            HIPCHK(h, hfx_launch_jk_rows(c.cb, c.ck, prec == 1, c.narrow, r, c.nblk[prec], s));
         }
     }
     for (int i = 0; i + 1 < ns; i++) {

@@ -322,9 +322,11 @@ static constexpr int HFX_TILE_COLS = 64;
 hipError_t hfx_launch_jk(int cls_bra, int cls_ket, bool fp32, const JKArgs& a, int ntiles, hipStream_t s);
 // LDS rows per bra row (nrow) and J_ab entries (nab) of a row-mode launch:
 // kind 0 = J/K in FP64, 1 = J/K in FP32, 2 = gradient in FP64, 3 = gradient
-// in FP32 (their slicing differs)
+// in FP32, 4 / 5 = narrow J/K in FP64 / FP32 (their slicing differs)
 void hfx_row_shape(int cls_bra, int cls_ket, int kind, int* nrow, int* nab);
-hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, const RowArgs& a, int nblocks, hipStream_t s);
+// narrow: the launch sliced by split_a_rows / split_c_rows (eri_rys.h)
+hipError_t hfx_launch_jk_rows(int cls_bra, int cls_ket, bool fp32, bool narrow, const RowArgs& a, int nblocks,
+                              hipStream_t s);
 hipError_t hfx_launch_grad(int cls_bra, int cls_ket, bool fp32, const GradArgs& a, int nblocks, hipStream_t s);
 // LDS bytes of a row-mode gradient launch (density rows as in hfx_row_lds, then 3 nat doubles)
 inline size_t hfx_grad_row_lds(int nrow, int ncols, int R, int dlds, bool fp32, int nat) {

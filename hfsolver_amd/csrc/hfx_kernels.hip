@@ -141,12 +141,13 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a) {
 #ifndef HFX_JAB_REG_MAX
 #define HFX_JAB_REG_MAX 36
 #endif
-// Row mode: one block = R bra shell pairs x a chunk of the ket table (see RowArgs).
-template <int LA, int LB, int LC, int LD, typename R>
+// Row mode: one block = R bra shell pairs x a chunk of the ket table (see RowArgs);
+// NARROW: the slicing of split_a_rows / split_c_rows for large molecules.
+template <int LA, int LB, int LC, int LD, typename R, bool NARROW = false>
 __global__ __launch_bounds__(hfx_row_threads(pair_class(LA, LB), pair_class(LC, LD), sizeof(R) == 4))
 void jk_row_kernel(RowArgs a) {
     constexpr bool F32 = sizeof(R) == 4;
-    constexpr bool SPA = split_a(LA, LB, LC, LD, F32), SPC = split_c(LA, LB, LC, LD, F32);
+    constexpr bool SPA = split_a_rows(LA, LB, LC, LD, F32, NARROW), SPC = split_c_rows(LA, LB, LC, LD, F32, NARROW);
     using Q = QClass<LA, LB, LC, LD, SPA, SPC>;
     constexpr int NAt = Q::NAt, NB = Q::NB, NCt = Q::NCt, ND = Q::ND;
     constexpr int nrow = NAt + NB;             // LDS K rows per bra row
@@ -782,28 +783,37 @@ const jk_fn JK_TABLE[6][6] = HFX_TABLE(jk_xy);
 
 #if HFX_PART == 2 || HFX_PART == 3
 typedef HFX_REAL_T RowReal;
-template <int X, int Y>
+template <int X, int Y, bool NARROW = false>
 hipError_t rows_xy(const RowArgs& a, int nblocks, hipStream_t s) {
     constexpr bool F32 = sizeof(RowReal) == 4;
-    constexpr int NAt = split_a(HFX_CLS(X), HFX_CLS(Y), F32) ? 1 : ncart(pair_class_la(X));
+    constexpr int NAt = split_a_rows(HFX_CLS(X), HFX_CLS(Y), F32, NARROW) ? 1 : ncart(pair_class_la(X));
     constexpr int NB = ncart(pair_class_lb(X));
     const size_t lds = hfx_row_lds(NAt + NB, NAt * NB, a.ncols, a.R, a.dlds, F32);
     if (lds > HFX_ROW_LDS_MAX || a.R < 1 || a.R > HFX_ROWS_MAX || a.kt.S < 1 || a.kt.S > HFX_SUB_MAX)
         return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>,
+        hipError_t e = hipFuncSetAttribute((const void*)jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal, NARROW>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, HFX_ROW_LDS_MAX);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    dim3 grid((unsigned)nblocks, (unsigned)nsplit(HFX_CLS(X), HFX_CLS(Y), F32));
-    hipLaunchKernelGGL((jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal>), grid, dim3(hfx_row_threads(X, Y, F32)), lds,
-                       s, a);
+    dim3 grid((unsigned)nblocks, (unsigned)nsplit_rows(HFX_CLS(X), HFX_CLS(Y), F32, NARROW));
+    hipLaunchKernelGGL((jk_row_kernel<HFX_CLS(X), HFX_CLS(Y), RowReal, NARROW>), grid,
+                       dim3(hfx_row_threads(X, Y, F32)), lds, s, a);
     return hipGetLastError();
+}
+// the narrow form where it differs (no second instantiation elsewhere)
+template <int X, int Y>
+hipError_t rows_narrow_xy(const RowArgs& a, int nblocks, hipStream_t s) {
+    if constexpr (narrow_differs(HFX_CLS(X), HFX_CLS(Y), sizeof(RowReal) == 4))
+        return rows_xy<X, Y, true>(a, nblocks, s);
+    else
+        return rows_xy<X, Y, false>(a, nblocks, s);
 }
 typedef hipError_t (*rows_fn)(const RowArgs&, int, hipStream_t);
 const rows_fn ROWS_TABLE[6][6] = HFX_TABLE(rows_xy);
+const rows_fn ROWS_NARROW_TABLE[6][6] = HFX_TABLE(rows_narrow_xy);
 #endif
 
 #if HFX_PART == 4 || HFX_PART == 5
@@ -862,7 +872,10 @@ void row_shape_xy(int kind, int* nrow, int* nab) {
     constexpr int NAt32 = split_a(HFX_CLS(X), HFX_CLS(Y), true) ? 1 : NA;
     constexpr int NAtg64 = gsplit_a(HFX_CLS(X), HFX_CLS(Y), false) ? 1 : NA;
     constexpr int NAtg32 = gsplit_a(HFX_CLS(X), HFX_CLS(Y), true) ? 1 : NA;
-    const int NAt = kind == 0 ? NAt64 : kind == 1 ? NAt32 : kind == 2 ? NAtg64 : NAtg32;
+    constexpr int NAtn64 = split_a_rows(HFX_CLS(X), HFX_CLS(Y), false, true) ? 1 : NA;
+    constexpr int NAtn32 = split_a_rows(HFX_CLS(X), HFX_CLS(Y), true, true) ? 1 : NA;
+    const int NAt = kind == 0 ? NAt64 : kind == 1 ? NAt32 : kind == 2 ? NAtg64 : kind == 3 ? NAtg32
+                  : kind == 4 ? NAtn64 : NAtn32;
     *nrow = NAt + NB;
     *nab = NAt * NB;
 }
@@ -886,15 +899,15 @@ hipError_t hfx_launch_jk(int cb, int ck, bool fp32, const JKArgs& a, int ntiles,
 #endif
 
 #if HFX_PART == 2
-hipError_t hfx_launch_jk_rows_f64(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s) {
+hipError_t hfx_launch_jk_rows_f64(int cb, int ck, bool narrow, const RowArgs& a, int nblocks, hipStream_t s) {
     HFX_CHECK_COMBO(cb, ck);
-    return ROWS_TABLE[cb][ck](a, nblocks, s);
+    return (narrow ? ROWS_NARROW_TABLE : ROWS_TABLE)[cb][ck](a, nblocks, s);
 }
 #endif
 #if HFX_PART == 3
-hipError_t hfx_launch_jk_rows_f32(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s) {
+hipError_t hfx_launch_jk_rows_f32(int cb, int ck, bool narrow, const RowArgs& a, int nblocks, hipStream_t s) {
     HFX_CHECK_COMBO(cb, ck);
-    return ROWS_TABLE[cb][ck](a, nblocks, s);
+    return (narrow ? ROWS_NARROW_TABLE : ROWS_TABLE)[cb][ck](a, nblocks, s);
 }
 #endif
 #if HFX_PART == 4
@@ -919,8 +932,8 @@ hipError_t hfx_launch_grad_rows_f32(int cb, int ck, const GradRowArgs& a, int nb
 #endif
 
 #if HFX_PART == 6
-hipError_t hfx_launch_jk_rows_f64(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s);
-hipError_t hfx_launch_jk_rows_f32(int cb, int ck, const RowArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_jk_rows_f64(int cb, int ck, bool narrow, const RowArgs& a, int nblocks, hipStream_t s);
+hipError_t hfx_launch_jk_rows_f32(int cb, int ck, bool narrow, const RowArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad_f64(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad_f32(int cb, int ck, const GradArgs& a, int nblocks, hipStream_t s);
 hipError_t hfx_launch_grad_rows_f64(int cb, int ck, const GradRowArgs& a, int nblocks, hipStream_t s);
@@ -932,13 +945,14 @@ int hfx_nsplit(int cb, int ck) {
 
 void hfx_row_shape(int cb, int ck, int kind, int* nrow, int* nab) {
     *nrow = *nab = 0;
-    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || kind < 0 || kind > 3) return;
+    if (cb < 0 || cb > 5 || ck < 0 || ck > cb || kind < 0 || kind > 5) return;
     SHAPE_TABLE[cb][ck](kind, nrow, nab);
 }
 
-hipError_t hfx_launch_jk_rows(int cb, int ck, bool fp32, const RowArgs& a, int nblocks, hipStream_t s) {
+hipError_t hfx_launch_jk_rows(int cb, int ck, bool fp32, bool narrow, const RowArgs& a, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipErrorInvalidValue;
-    return fp32 ? hfx_launch_jk_rows_f32(cb, ck, a, nblocks, s) : hfx_launch_jk_rows_f64(cb, ck, a, nblocks, s);
+    return fp32 ? hfx_launch_jk_rows_f32(cb, ck, narrow, a, nblocks, s)
+                : hfx_launch_jk_rows_f64(cb, ck, narrow, a, nblocks, s);
 }
 
 hipError_t hfx_launch_grad(int cb, int ck, bool fp32, const GradArgs& a, int nblocks, hipStream_t s) {
