@@ -13,8 +13,10 @@
 //      primitive count); for every class combination each bra row's range of
 //      kets per sublist with Q_ab Q_cd >= schwarz_threshold, split into an
 //      FP64 and an FP32 share at fp32_threshold, the quartet counts, and the
-//      row-mode block list; the host sizes the allocations from counts and
-//      builds tile lists only where the LDS rows do not fit.
+//      row-mode block list; the host sizes the allocations from counts. A
+//      combination whose K rows do not fit in LDS runs as narrow row launches
+//      (sliced over the a-components); tile lists are built only where even
+//      those do not fit.
 // Per Fock build (hfx_fock_2e): P -> scaled symmetric D, D_cd gathered per ket,
 // one launch per class combination and precision, J_cd scattered, then J, K
 // symmetrised and unscaled, G = J - K/2. One-electron integrals and their
