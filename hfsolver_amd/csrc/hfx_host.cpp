@@ -82,7 +82,11 @@ struct hfx_engine {
     // row-mode LDS per block (a class whose K rows need more runs as tiles; the
     // heaviest classes run one block per CU)
     size_t lds_budget = HFX_ROW_LDS_MAX;
-    size_t dlds_budget = 120 * 1024;  // largest row-mode LDS per block with the density rows staged
+    // largest row-mode LDS per block with the density rows staged, FP64 and FP32
+    // launches (single-precision rows: the FP32 launches gain from staging them
+    // up to the whole budget, the FP64 launches lose above 120 KB)
+    size_t dlds_budget = 120 * 1024;
+    size_t dlds_budget32 = HFX_ROW_LDS_MAX;
     // pairs
     std::vector<ShellPair> pairs;
     std::vector<PrimPair> prims;
@@ -117,7 +121,7 @@ struct hfx_engine {
     // row-mode J/K of one class combination over the whole ket table
     struct RowCombo {
         int cb, ck;
-        int R = 1, dlds = 0, ncols = 0, offd = 0;
+        int R = 1, dlds = 0, ncols = 0, offd = 0;   // dlds: bit 0 FP64 launch, bit 1 FP32 launch
         bool narrow = false;   // launches sliced by split_a_rows / split_c_rows (K rows too wide otherwise)
         int nblk[2] = {0, 0};
         int chunk[2] = {0, 0};
@@ -653,16 +657,28 @@ int build_lists(hfx_t* h) {
             rc.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
             rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
             if (rc.ncols >= 32768) return fail(h, "ket table: too many functions of one angular momentum");
-            // the FP64 and FP32 launches slice differently: plan for the larger LDS of the two
+            // the FP64 and FP32 launches slice differently and decide the staging of
+            // the density rows each within its own budget (dlds bit 0: the FP64
+            // launch, bit 1: the FP32 launch); the blocks are planned for the
+            // larger LDS of the two
             int nrow64 = 0, nab64 = 0, nrow32 = 0, nab32 = 0;
             hfx_row_shape(cb, ck, 0, &nrow64, &nab64);
             hfx_row_shape(cb, ck, 1, &nrow32, &nab32);
-            auto row_lds = [&](int R, int dl) {
-                return std::max(hfx_row_lds(nrow64, nab64, rc.ncols, R, dl, false),
-                                hfx_row_lds(nrow32, nab32, rc.ncols, R, dl, true));
+            auto staging = [&]() {
+                int m = 0;
+                if (h->row_dlds) {
+                    if (hfx_row_lds(nrow64, nab64, rc.ncols, 1, 1, false) <= std::min(h->lds_budget, h->dlds_budget))
+                        m |= 1;
+                    if (hfx_row_lds(nrow32, nab32, rc.ncols, 1, 1, true) <= std::min(h->lds_budget, h->dlds_budget32))
+                        m |= 2;
+                }
+                return m;
             };
-            int dlds = h->row_dlds;
-            if (dlds && row_lds(1, 1) > std::min(h->lds_budget, h->dlds_budget)) dlds = 0;
+            auto row_lds = [&](int R, int m) {
+                return std::max(hfx_row_lds(nrow64, nab64, rc.ncols, R, m & 1, false),
+                                hfx_row_lds(nrow32, nab32, rc.ncols, R, (m >> 1) & 1, true));
+            };
+            int dlds = staging();
             const size_t need = row_lds(1, dlds);
             // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
             int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
@@ -671,8 +687,7 @@ int build_lists(hfx_t* h) {
                 // narrow launches: one K row of the a-shell per bra, one row per block
                 hfx_row_shape(cb, ck, 4, &nrow64, &nab64);
                 hfx_row_shape(cb, ck, 5, &nrow32, &nab32);
-                dlds = h->row_dlds;
-                if (dlds && row_lds(1, 1) > std::min(h->lds_budget, h->dlds_budget)) dlds = 0;
+                dlds = staging();
                 if (row_lds(1, dlds) <= h->lds_budget) {
                     R = 1;
                     rc.narrow = true;
@@ -989,6 +1004,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
     if (const char* e = std::getenv("HFX_ROW_NARROW")) h->row_narrow = std::atoi(e) != 0;
     if (const char* e = std::getenv("HFX_ROW_DLDS_KB")) h->dlds_budget = (size_t)std::max(1, std::atoi(e)) * 1024;
+    if (const char* e = std::getenv("HFX_ROW_DLDS32_KB")) h->dlds_budget32 = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_BLOCKS")) h->row_blocks = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("HFX_STREAMS")) h->nstreams = std::max(1, std::min(HFX_AUX_STREAMS + 1, std::atoi(e)));
     h->nat = nat;
@@ -1089,7 +1105,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.offd = c.offd;
             r.bf_c = h->d_bfL[pair_class_la(c.ck)];
             r.bf_d = h->d_bfL[pair_class_lb(c.ck)];
-            r.dlds = c.dlds && hfx_row_stage_density(c.cb, c.ck, prec == 1);
+            r.dlds = ((c.dlds >> prec) & 1) && hfx_row_stage_density(c.cb, c.ck, prec == 1);
             r.D = h->d_D;
             r.J = h->d_Jb;
             r.K = h->d_Kb;
@@ -1289,7 +1305,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         const int prec = o.second % 2;
         int nrow = 0, nab = 0;
         hfx_row_shape(c.cb, c.ck, prec ? 3 : 2, &nrow, &nab);   // the launch's slicing (FP64 or FP32)
-        int dlds = c.dlds;
+        int dlds = (c.dlds >> prec) & 1;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) dlds = 0;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) continue;
         int si = 0;
