@@ -1305,7 +1305,9 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         const int prec = o.second % 2;
         int nrow = 0, nab = 0;
         hfx_row_shape(c.cb, c.ck, prec ? 3 : 2, &nrow, &nab);   // the launch's slicing (FP64 or FP32)
-        int dlds = (c.dlds >> prec) & 1;
+        // the derivative launch holds no K rows: it stages the density rows
+        // whenever they fit, whatever the J/K launch of the combination does
+        int dlds = h->row_dlds;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) dlds = 0;
         if (hfx_grad_row_lds(nrow, c.ncols, c.R, dlds, false, h->nat) > HFX_ROW_LDS_MAX) continue;
         int si = 0;
