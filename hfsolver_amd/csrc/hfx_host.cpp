@@ -79,8 +79,9 @@ struct hfx_engine {
     std::vector<int> bf_atom;
     std::vector<int> bfL[HFX_LMAX + 1];   // functions of the shells of each L, in shell order
     int* d_bfL[HFX_LMAX + 1] = {nullptr};
-    // row-mode LDS per block (a class whose K rows need more runs as tiles; the
-    // heaviest classes run one block per CU)
+    // row-mode LDS per block (a combination whose K rows need more runs as
+    // narrow rows or, failing those, as tiles; the heaviest classes run one
+    // block per CU)
     size_t lds_budget = HFX_ROW_LDS_MAX;
     // largest row-mode LDS per block with the density rows staged, FP64 and FP32
     // launches (single-precision rows: the FP32 launches gain from staging them
