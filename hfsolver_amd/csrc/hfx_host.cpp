@@ -13,10 +13,12 @@
 //      primitive count); for every class combination each bra row's range of
 //      kets per sublist with Q_ab Q_cd >= schwarz_threshold, split into an
 //      FP64 and an FP32 share at fp32_threshold, the quartet counts, and the
-//      row-mode block list; the host sizes the allocations from counts. A
-//      combination whose K rows do not fit in LDS runs as narrow row launches
-//      (sliced over the a-components); tile lists are built only where even
-//      those do not fit.
+//      row-mode block list; the host sizes the allocations from counts. Pair
+//      classes with many ket columns get one ket table per column window (the
+//      columns in the spatial order of their atoms), so that a row launch keeps
+//      only its window's columns in LDS; a combination whose K rows still do
+//      not fit runs as narrow row launches (sliced over the a-components); tile
+//      lists are built only where even those do not fit.
 // Per Fock build (hfx_fock_2e): P -> scaled symmetric D, D_cd gathered per ket,
 // one launch per class combination and precision, J_cd scattered, then J, K
 // symmetrised and unscaled, G = J - K/2. One-electron integrals and their
@@ -94,7 +96,7 @@ struct hfx_engine {
     std::vector<PrimExp> pexp;
     std::vector<int> cls_list[HFX_NPAIRCLASS];  // significant pairs, Q descending
     struct Combo {
-        int cb, ck;
+        int cb, ck, win = 0;
         int* d_ket = nullptr;   // ket sublist (equal primitive-pair count)
         int nket = 0;
         int npk = 0;            // its primitive-pair count
@@ -113,15 +115,28 @@ struct hfx_engine {
         int* d_gblk_k0[2] = {nullptr, nullptr};
     };
     std::vector<Combo> combos;
-    // ket tables of every pair class (sublists in ascending primitive-pair count)
+    // ket tables of every pair class and column window (sublists in ascending
+    // primitive-pair count)
     struct KetTable {
         KetTab t{};
         int ncomp = 0, nd = 0;
     };
-    KetTable ktab[HFX_NPAIRCLASS];
+    KetTable ktab[HFX_NPAIRCLASS][HFX_WIN_MAX];
+    // column windows (large molecules): window w of pair class c holds the
+    // pairs whose C shell lies in the w-th of nwin[c] equal ranges of the L_C
+    // columns, which then follow the atoms in space (Morton order); a row
+    // launch over one window keeps only that window's columns in LDS. The
+    // LDS columns of window w: [win_c0, win_c0 + win_cn) of the L_C functions,
+    // then [win_d0, win_d0 + win_dn) of the L_D functions (L_D != L_C), or one
+    // range [win_c0, win_c0 + win_cn) holding both (L_D == L_C).
+    int win_cols = 1280;   // widest column range before a class is cut into windows (HFX_ROW_WINDOW_COLS, 0: never)
+    int nwin[HFX_NPAIRCLASS] = {1, 1, 1, 1, 1, 1};
+    int win_c0[HFX_NPAIRCLASS][HFX_WIN_MAX] = {}, win_cn[HFX_NPAIRCLASS][HFX_WIN_MAX] = {};
+    int win_d0[HFX_NPAIRCLASS][HFX_WIN_MAX] = {}, win_dn[HFX_NPAIRCLASS][HFX_WIN_MAX] = {};
     // row-mode J/K of one class combination over the whole ket table
     struct RowCombo {
-        int cb, ck;
+        int cb, ck, win = 0;
+        int col0_c = 0, col0_d = 0;   // the window's first LDS columns (RowArgs)
         int R = 1, dlds = 0, ncols = 0, offd = 0;   // dlds: bit 0 FP64 launch, bit 1 FP32 launch
         bool narrow = false;   // launches sliced by split_a_rows / split_c_rows (K rows too wide otherwise)
         int nblk[2] = {0, 0};
@@ -252,7 +267,8 @@ void free_geometry(hfx_t* h) {
     h->geom_allocs.clear();
     h->combos.clear();
     h->rcombos.clear();
-    for (int c = 0; c < HFX_NPAIRCLASS; c++) h->ktab[c] = hfx_engine::KetTable{};
+    for (int c = 0; c < HFX_NPAIRCLASS; c++)
+        for (int w = 0; w < HFX_WIN_MAX; w++) h->ktab[c][w] = hfx_engine::KetTable{};
     h->d_pairs = nullptr;
     h->d_prims = nullptr;
     h->d_primsf = nullptr;
@@ -326,8 +342,45 @@ int build_shells(hfx_t* h, int nbf, const int* nprim, const int* istart, const d
         i += nc;
     }
     h->nbf = nbf;
+    // column windows per pair class, from the widths of its ket rows
+    bool windows = false;
+    for (int c = 0; c < HFX_NPAIRCLASS; c++) {
+        const int lc = pair_class_la(c), ld = pair_class_lb(c);
+        int ncols = 0;
+        for (const Shell& sh : h->shells)
+            if (sh.L == lc || (sh.L == ld && ld != lc)) ncols += ncart(sh.L);
+        int W = h->win_cols > 0 ? (ncols + h->win_cols - 1) / h->win_cols : 1;
+        h->nwin[c] = std::max(1, std::min(W, HFX_WIN_MAX));
+        windows = windows || h->nwin[c] > 1;
+    }
+    // LDS columns: the functions of the shells of each L, in shell order, or
+    // with windows in the Morton order of their atoms (10 bits per axis over
+    // the molecule's bounding box), so that a window's pairs stay close in space
+    std::vector<int> order(h->shells.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    if (windows) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int a = 0; a < h->nat; a++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], h->xyz[3 * a + k]);
+                hi[k] = std::max(hi[k], h->xyz[3 * a + k]);
+            }
+        std::vector<unsigned long long> key(h->nat);
+        for (int a = 0; a < h->nat; a++) {
+            unsigned long long m = 0;
+            for (int k = 0; k < 3; k++) {
+                const double span = hi[k] - lo[k];
+                const unsigned q = span > 0 ? (unsigned)std::min(1023.0, (h->xyz[3 * a + k] - lo[k]) / span * 1023.0) : 0u;
+                for (int b = 0; b < 10; b++) m |= (unsigned long long)((q >> b) & 1u) << (3 * b + k);
+            }
+            key[a] = m;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return key[h->shells[x].atom] < key[h->shells[y].atom]; });
+    }
     for (int L = 0; L <= HFX_LMAX; L++) h->bfL[L].clear();
-    for (Shell& sh : h->shells) {
+    for (int i : order) {
+        Shell& sh = h->shells[i];
         sh.cpos = (int)h->bfL[sh.L].size();
         for (int c = 0; c < ncart(sh.L); c++) h->bfL[sh.L].push_back(sh.f0 + c);
     }
@@ -526,9 +579,9 @@ int np_bucket(int np) {
 // bucket order, sub0 = the class's first entry in that order. aux receives the
 // per-ket arrays of the range search (bound, class position, primitive counts
 // and their prefix sums).
-int build_ket_table(hfx_t* h, int c, const std::vector<std::array<int, 2>>& subs, int sub0, const PairOrder& po,
-                    KetAux& aux) {
-    hfx_engine::KetTable& T = h->ktab[c];
+int build_ket_table(hfx_t* h, int c, int w, const std::vector<std::array<int, 2>>& subs, int sub0,
+                    const PairOrder& po, KetAux& aux) {
+    hfx_engine::KetTable& T = h->ktab[c][w];
     T = hfx_engine::KetTable{};
     KetTab& t = T.t;
     const int S = (int)subs.size();
@@ -592,11 +645,60 @@ int build_lists(hfx_t* h) {
     h->rcombos.clear();
     h->npairs_sig = 0;
     if (np == 0) return 0;
+    // column window of every pair, and the LDS columns of every window (over all
+    // pairs of the window, significant or not)
+    int* d_pwin = nullptr;
+    {
+        std::vector<int> pwin(np, 0);
+        bool windows = false;
+        int lo[HFX_NPAIRCLASS][HFX_WIN_MAX], hi[HFX_NPAIRCLASS][HFX_WIN_MAX];
+        int dlo[HFX_NPAIRCLASS][HFX_WIN_MAX], dhi[HFX_NPAIRCLASS][HFX_WIN_MAX];
+        for (int c = 0; c < HFX_NPAIRCLASS; c++)
+            for (int w = 0; w < HFX_WIN_MAX; w++) {
+                lo[c][w] = dlo[c][w] = 1 << 30;
+                hi[c][w] = dhi[c][w] = 0;
+            }
+        for (int i = 0; i < np; i++) {
+            const ShellPair& sp = h->pairs[i];
+            const int c = sp.cls, W = h->nwin[c];
+            if (W == 1) continue;
+            windows = true;
+            const int lc = pair_class_la(c), ld = pair_class_lb(c);
+            const int w = std::min(W - 1, (int)((long long)sp.cpa * W / (long long)h->bfL[lc].size()));
+            pwin[i] = w;
+            if (ld != lc) {
+                lo[c][w] = std::min(lo[c][w], sp.cpa);
+                hi[c][w] = std::max(hi[c][w], sp.cpa + ncart(lc));
+                dlo[c][w] = std::min(dlo[c][w], sp.cpb);
+                dhi[c][w] = std::max(dhi[c][w], sp.cpb + ncart(ld));
+            } else {
+                lo[c][w] = std::min(lo[c][w], std::min(sp.cpa, sp.cpb));
+                hi[c][w] = std::max(hi[c][w], std::max(sp.cpa, sp.cpb) + ncart(lc));
+            }
+        }
+        for (int c = 0; c < HFX_NPAIRCLASS; c++) {
+            const int lc = pair_class_la(c), ld = pair_class_lb(c);
+            for (int w = 0; w < h->nwin[c]; w++) {
+                if (h->nwin[c] == 1) {
+                    h->win_c0[c][w] = 0;
+                    h->win_cn[c][w] = (int)h->bfL[lc].size();
+                    h->win_d0[c][w] = 0;
+                    h->win_dn[c][w] = ld != lc ? (int)h->bfL[ld].size() : 0;
+                    continue;
+                }
+                h->win_c0[c][w] = hi[c][w] > lo[c][w] ? lo[c][w] : 0;
+                h->win_cn[c][w] = std::max(0, hi[c][w] - lo[c][w]);
+                h->win_d0[c][w] = dhi[c][w] > dlo[c][w] ? dlo[c][w] : 0;
+                h->win_dn[c][w] = std::max(0, dhi[c][w] - dlo[c][w]);
+            }
+        }
+        if (windows && upload(h, &d_pwin, pwin, true)) return 1;
+    }
     PairOrder po;
     if (dmalloc(h, (void**)&po.cls_order, (size_t)np * sizeof(int), true)) return 1;
     if (dmalloc(h, (void**)&po.sub_order, (size_t)np * sizeof(int), true)) return 1;
     if (dmalloc(h, (void**)&po.pos, (size_t)np * sizeof(int), true)) return 1;
-    HIPCHK(h, hfx_pair_order(h->d_pairs, np, thr, po, h->stream));
+    HIPCHK(h, hfx_pair_order(h->d_pairs, d_pwin, np, thr, po, h->stream));
     h->npairs_sig = po.nsig;
     // class lists: views of the sorted array on the device, a host copy for the counts and the tile lists
     std::vector<int> order(po.nsig), hsub;
@@ -608,33 +710,34 @@ int build_lists(hfx_t* h) {
         h->cls_list[c].assign(order.begin() + cls_off[c], order.begin() + cls_off[c + 1]);
         if (po.ncls[c] > 0) h->d_cls_list[c] = po.cls_order + cls_off[c];
     }
-    // sublists (bucket, count) of every class, in the (class, bucket) order
-    std::vector<std::array<int, 2>> subs[HFX_NPAIRCLASS];
-    int sub0[HFX_NPAIRCLASS] = {0};
+    // sublists (bucket, count) of every class and window, in the (class, window, bucket) order
+    std::vector<std::array<int, 2>> subs[HFX_NPAIRCLASS][HFX_WIN_MAX];
+    int sub0[HFX_NPAIRCLASS][HFX_WIN_MAX] = {};
     {
         int off = 0;
         for (const auto& r : po.runs) {
-            if (subs[r[0]].empty()) sub0[r[0]] = off;
-            subs[r[0]].push_back({r[1], r[2]});
-            off += r[2];
+            if (subs[r[0]][r[1]].empty()) sub0[r[0]][r[1]] = off;
+            subs[r[0]][r[1]].push_back({r[2], r[3]});
+            off += r[3];
         }
     }
-    KetAux aux[HFX_NPAIRCLASS] = {};
+    KetAux aux[HFX_NPAIRCLASS][HFX_WIN_MAX] = {};
     const auto tk = std::chrono::steady_clock::now();
-    for (int c = 0; c < HFX_NPAIRCLASS; c++) {
-        if (subs[c].empty()) continue;
-        if ((int)subs[c].size() > HFX_SUB_MAX)
-            return fail(h, "pair class %d has %zu primitive-count sublists (at most %d)", c, subs[c].size(),
-                        HFX_SUB_MAX);
-        if (build_ket_table(h, c, subs[c], sub0[c], po, aux[c])) return 1;
-    }
+    for (int c = 0; c < HFX_NPAIRCLASS; c++)
+        for (int w = 0; w < h->nwin[c]; w++) {
+            if (subs[c][w].empty()) continue;
+            if ((int)subs[c][w].size() > HFX_SUB_MAX)
+                return fail(h, "pair class %d has %zu primitive-count sublists (at most %d)", c, subs[c][w].size(),
+                            HFX_SUB_MAX);
+            if (build_ket_table(h, c, w, subs[c][w], sub0[c][w], po, aux[c][w])) return 1;
+        }
     if (h->opts.verbose) {
         HIPCHK(h, hipStreamSynchronize(h->stream));
         fprintf(stderr, "hfx: class lists and ket tables %.3f s\n", seconds_since(tk));
     }
     // per class combination: row-mode plan, then the ket ranges of every bra row on the GPU
     struct Pending {
-        int cb, ck, nb, S, R, dlds;
+        int cb, ck, w, nb, S, R, dlds;
         bool rowmode;
         size_t stat;          // first entry of its statistics
         int* rkb[2];
@@ -644,20 +747,27 @@ int build_lists(hfx_t* h) {
     std::vector<Pending> pend;
     size_t nstat = 0;
     for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
-        for (int ck = 0; ck <= cb; ck++) {
-            if (po.ncls[cb] == 0 || po.ncls[ck] == 0) continue;
+        for (int ck = 0; ck <= cb; ck++)
+            for (int w = 0; w < h->nwin[ck]; w++) {
+            if (po.ncls[cb] == 0 || subs[ck][w].empty()) continue;
             Pending q{};
             q.cb = cb;
             q.ck = ck;
+            q.w = w;
             q.nb = po.ncls[cb];
-            q.S = (int)subs[ck].size();
+            q.S = (int)subs[ck][w].size();
             hfx_engine::RowCombo& rc = q.rc;
             rc.cb = cb;
             rc.ck = ck;
+            rc.win = w;
             const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
-            rc.ncols = (int)h->bfL[lc].size() + (ld != lc ? (int)h->bfL[ld].size() : 0);
-            rc.offd = ld != lc ? (int)h->bfL[lc].size() : 0;
-            if (rc.ncols >= 32768) return fail(h, "ket table: too many functions of one angular momentum");
+            if ((int)h->bfL[lc].size() >= 32768 || (int)h->bfL[ld].size() >= 32768)
+                return fail(h, "ket table: too many functions of one angular momentum");
+            // LDS columns: the window's (all columns without windows)
+            rc.ncols = h->win_cn[ck][w] + (ld != lc ? h->win_dn[ck][w] : 0);
+            rc.offd = ld != lc ? h->win_cn[ck][w] : 0;
+            rc.col0_c = h->win_c0[ck][w];
+            rc.col0_d = ld != lc ? h->win_d0[ck][w] : h->win_c0[ck][w];
             // the FP64 and FP32 launches slice differently and decide the staging of
             // the density rows each within its own budget (dlds bit 0: the FP64
             // launch, bit 1: the FP32 launch); the blocks are planned for the
@@ -716,8 +826,8 @@ int build_lists(hfx_t* h) {
         RangeArgs a{};
         a.pairs = h->d_pairs;
         a.bra_list = h->d_cls_list[q.cb];
-        a.kt = h->ktab[q.ck].t;
-        a.x = aux[q.ck];
+        a.kt = h->ktab[q.ck][q.w].t;
+        a.x = aux[q.ck][q.w];
         a.nb = q.nb;
         a.S = q.S;
         a.same = q.cb == q.ck;
@@ -750,7 +860,7 @@ int build_lists(hfx_t* h) {
     }
     for (size_t i = 0; i < pend.size(); i++) {
         Pending& q = pend[i];
-        const int cb = q.cb, ck = q.ck, nb = q.nb, S = q.S;
+        const int cb = q.cb, ck = q.ck, w = q.w, nb = q.nb, S = q.S;
         std::vector<int> hkb[2], hke[2];
         if (!q.rowmode)
             for (int prec = 0; prec < 2; prec++) {
@@ -763,8 +873,9 @@ int build_lists(hfx_t* h) {
             hfx_engine::Combo cmb;
             cmb.cb = cb;
             cmb.ck = ck;
-            cmb.nket = subs[ck][s][1];
-            cmb.npk = subs[ck][s][0];
+            cmb.win = w;
+            cmb.nket = subs[ck][w][s][1];
+            cmb.npk = subs[ck][w][s][0];
             cmb.rowmode = q.rowmode;
             const double* st = stats.data() + q.stat + 4 * (size_t)s;
             cmb.nq[0] = st[0];
@@ -775,8 +886,8 @@ int build_lists(hfx_t* h) {
                 for (int prec = 0; prec < 2; prec++) cmb.ntiles[prec] = cmb.nq[prec] > 0 ? 1 : 0;   // statistics only
             } else {
                 // tile mode: the sublist's ket list, per-row ranges relative to it, tiles, gradient blocks
-                const int soff = h->ktab[ck].t.off[s];
-                std::vector<int> kl(hsub.begin() + sub0[ck] + soff, hsub.begin() + sub0[ck] + soff + cmb.nket);
+                const int soff = h->ktab[ck][w].t.off[s];
+                std::vector<int> kl(hsub.begin() + sub0[ck][w] + soff, hsub.begin() + sub0[ck][w] + soff + cmb.nket);
                 if (upload(h, &cmb.d_ket, kl, true)) return 1;
                 for (int prec = 0; prec < 2; prec++) {
                     std::vector<int> kb(nb), ke(nb), ti0, tj0;
@@ -864,9 +975,10 @@ int build_lists(hfx_t* h) {
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck),
                     c.npk, c.rowmode ? "rows" : "tiles", c.nq[0], c.nq[1], c.nprimq[0], c.nprimq[1]);
         for (auto& c : h->rcombos)
-            fprintf(stderr, "hfx:   rows (%d%d|%d%d): R=%d dlds=%d narrow=%d ncols=%d blocks %d / %d chunk %d / %d\n",
-                    pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck), c.R, c.dlds,
-                    (int)c.narrow, c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
+            fprintf(stderr,
+                    "hfx:   rows (%d%d|%d%d) window %d/%d: R=%d dlds=%d narrow=%d ncols=%d blocks %d / %d chunk %d / %d\n",
+                    pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck), c.win,
+                    h->nwin[c.ck], c.R, c.dlds, (int)c.narrow, c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
     }
     return 0;
 }
@@ -1004,6 +1116,7 @@ int hfx_create(int nbf, const int* nprim, const int* istart, const double* x, co
     if (const char* e = std::getenv("HFX_ROW_PAIR_KB")) h->row_pair_bytes = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_DLDS")) h->row_dlds = std::atoi(e) != 0;
     if (const char* e = std::getenv("HFX_ROW_NARROW")) h->row_narrow = std::atoi(e) != 0;
+    if (const char* e = std::getenv("HFX_ROW_WINDOW_COLS")) h->win_cols = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("HFX_ROW_DLDS_KB")) h->dlds_budget = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_DLDS32_KB")) h->dlds_budget32 = (size_t)std::max(1, std::atoi(e)) * 1024;
     if (const char* e = std::getenv("HFX_ROW_BLOCKS")) h->row_blocks = std::max(1, std::atoi(e));
@@ -1061,8 +1174,10 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     if (h->dthr > 0)
         HIPCHK(h, hfx_launch_shell_dmax(h->d_D, n, h->d_dsh_f0, h->d_dsh_nc, nsh, h->d_dsh, h->stream));
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        if (h->ktab[c].t.nk > 0)
-            HIPCHK(h, hfx_launch_ket_gather(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_D, n, h->stream));
+        for (int w = 0; w < h->nwin[c]; w++)
+            if (h->ktab[c][w].t.nk > 0)
+                HIPCHK(h, hfx_launch_ket_gather(h->ktab[c][w].t, h->ktab[c][w].ncomp, h->ktab[c][w].nd, h->d_D, n,
+                                                h->stream));
     // class launches dealt round-robin over the main and the extra streams,
     // which start after the gathers and are joined before the scatters
     const int ns = h->nstreams;
@@ -1094,7 +1209,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.prims = h->d_prims;
             r.primsf = h->d_primsf;
             r.bra_list = h->d_cls_list[c.cb];
-            r.kt = h->ktab[c.ck].t;
+            r.kt = h->ktab[c.ck][c.win].t;
             r.kbeg = c.d_kbeg[prec];
             r.kend = c.d_kend[prec];
             r.blk_row = c.d_blk_row[prec];
@@ -1106,6 +1221,8 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
             r.offd = c.offd;
             r.bf_c = h->d_bfL[pair_class_la(c.ck)];
             r.bf_d = h->d_bfL[pair_class_lb(c.ck)];
+            r.col0_c = c.col0_c;
+            r.col0_d = c.col0_d;
             r.dlds = ((c.dlds >> prec) & 1) && hfx_row_stage_density(c.cb, c.ck, prec == 1);
             r.D = h->d_D;
             r.J = h->d_Jb;
@@ -1164,8 +1281,10 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
         }
     }
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        if (h->ktab[c].t.nk > 0)
-            HIPCHK(h, hfx_launch_ket_scatter(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_Jb, n, h->stream));
+        for (int w = 0; w < h->nwin[c]; w++)
+            if (h->ktab[c][w].t.nk > 0)
+                HIPCHK(h, hfx_launch_ket_scatter(h->ktab[c][w].t, h->ktab[c][w].ncomp, h->ktab[c][w].nd, h->d_Jb, n,
+                                                 h->stream));
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     HIPCHK(h, hfx_launch_finish(h->d_Jb, h->d_Kb, h->d_scale, dG, dJ, dK, n, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1284,9 +1403,11 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     // row mode over the ket tables where the J/K launch is in row mode
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        if (h->ktab[c].t.nk > 0)
-            HIPCHK(h, hfx_launch_ket_gather(h->ktab[c].t, h->ktab[c].ncomp, h->ktab[c].nd, h->d_D, n, h->stream));
-    bool done[HFX_NPAIRCLASS][HFX_NPAIRCLASS] = {};
+        for (int w = 0; w < h->nwin[c]; w++)
+            if (h->ktab[c][w].t.nk > 0)
+                HIPCHK(h, hfx_launch_ket_gather(h->ktab[c][w].t, h->ktab[c][w].ncomp, h->ktab[c][w].nd, h->d_D, n,
+                                                h->stream));
+    bool done[HFX_NPAIRCLASS][HFX_NPAIRCLASS][HFX_WIN_MAX] = {};
     // the row launches dealt largest-first over the streams, as in the Fock build
     const int ns = h->nstreams;
     if (ns > 1) {
@@ -1321,13 +1442,15 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         r.prims = h->d_prims;
         r.pexp = h->d_pexp;
         r.bra_list = h->d_cls_list[c.cb];
-        r.kt = h->ktab[c.ck].t;
+        r.kt = h->ktab[c.ck][c.win].t;
         r.R = c.R;
         r.nbra = (int)h->cls_list[c.cb].size();
         r.ncols = c.ncols;
         r.offd = c.offd;
         r.bf_c = h->d_bfL[pair_class_la(c.ck)];
         r.bf_d = h->d_bfL[pair_class_lb(c.ck)];
+        r.col0_c = c.col0_c;
+        r.col0_d = c.col0_d;
         r.dlds = dlds;
         r.bf_atom = h->d_bf_atom;
         r.nat = h->nat;
@@ -1340,7 +1463,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
         r.blk_k0 = c.d_blk_k0[prec];
         r.chunk = c.chunk[prec];
         HIPCHK(h, hfx_launch_grad_rows(c.cb, c.ck, prec == 1, r, c.nblk[prec], s));
-        done[c.cb][c.ck] = true;
+        done[c.cb][c.ck][c.win] = true;
     }
     for (int i = 0; i + 1 < ns; i++) {
         HIPCHK(h, hipEventRecord(h->evaux[i + 1], h->aux[i]));
@@ -1356,7 +1479,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     a.grad = h->d_grad;
     a.n = n;
     for (const auto& c : h->combos) {
-        if (done[c.cb][c.ck]) continue;
+        if (done[c.cb][c.ck][c.win]) continue;
         if (3 * (size_t)h->nat * sizeof(double) > HFX_GRAD_LDS_MAX)
             return fail(h, "hfx_grad_2e: %d atoms need %zu bytes of LDS in the fallback gradient kernel (at most %zu)",
                         h->nat, 3 * (size_t)h->nat * sizeof(double), HFX_GRAD_LDS_MAX);

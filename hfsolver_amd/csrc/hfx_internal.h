@@ -34,6 +34,7 @@ struct JKArgs {
 };
 
 static constexpr int HFX_SUB_MAX = 16;  // primitive-count sublists of a ket class
+static constexpr int HFX_WIN_MAX = 16;  // column windows of a ket class
 
 // Per-position record of a ket table: pair id, first function of C and of D,
 // and cp = cpa | cpb << 15 | (C == D) << 30 (cpa / cpb: LDS column of the
@@ -67,15 +68,16 @@ struct KetTab {
 
 // ---- quartet lists built on the GPU (hfx_lists.hip) ----
 // Significant pairs in class order (Q-descending within a class) and in
-// (class, primitive-count bucket) order; runs = (class, bucket, count) of the
-// second order; pos = position of every significant pair in its class list.
+// (class, column window, primitive-count bucket) order; runs = (class, window,
+// bucket, count) of the second order; pos = position of every significant pair
+// in its class list.
 struct PairOrder {
     int* cls_order = nullptr;   // [npairs] (device, allocated by the caller)
     int* sub_order = nullptr;   // [npairs]
     int* pos = nullptr;         // [npairs], -1 for pairs that are not significant
     int ncls[8] = {0};
     int nsig = 0;
-    std::vector<std::array<int, 3>> runs;
+    std::vector<std::array<int, 4>> runs;
 };
 // Mutable views of one class's ket table for the fill kernel.
 struct KetOut {
@@ -114,7 +116,9 @@ struct BlockArgsL {
     const int* rke;
     int nb, S, R, chunk;
 };
-hipError_t hfx_pair_order(const ShellPair* d_pairs, int np, double thr, PairOrder& o, hipStream_t s);
+// d_pwin: column window of every pair (nullptr: one window)
+hipError_t hfx_pair_order(const ShellPair* d_pairs, const int* d_pwin, int np, double thr, PairOrder& o,
+                          hipStream_t s);
 hipError_t hfx_launch_ket_fill(const KetTab& t, const KetOut& o, const KetAux& x, const int* sub_order,
                                const ShellPair* pairs, const PrimPair* prims, const PrimExp* pexp, const int* pos,
                                hipStream_t s);
@@ -131,7 +135,8 @@ hipError_t hfx_group_blocks(const BlockArgsL& a, int** brow, int** bk0, int* nbl
 // contributions of each row land in the rows of its shells A and B,
 // accumulated in LDS over all the block's kets and flushed once: LDS columns
 // are the functions of the L_C shells (then, when L_D != L_C, those of the
-// L_D shells from column offd on); bf_c / bf_d map columns back to basis
+// L_D shells from column offd on), or with column windows the window's range
+// of them (from col0_c / col0_d); bf_c / bf_d map columns back to basis
 // functions. With dlds the same rows of the density are staged in LDS in the
 // same layout. J_ab is reduced over the block; J_cd of each ket is summed over
 // the R rows and added to the ket table's accumulators.
@@ -152,6 +157,9 @@ struct RowArgs {
     int offd;
     const int* bf_c;
     const int* bf_d;
+    // column window of the ket table: LDS column 0 (offd) is position col0_c
+    // (col0_d) among the L_C (L_D) functions; both 0 without windows
+    int col0_c, col0_d;
     int dlds;       // stage the density rows of the bra shells in LDS
     const double* D;
     double* J;
@@ -220,6 +228,9 @@ struct GradRowArgs {
     int offd;
     const int* bf_c;
     const int* bf_d;
+    // column window of the ket table: LDS column 0 (offd) is position col0_c
+    // (col0_d) among the L_C (L_D) functions; both 0 without windows
+    int col0_c, col0_d;
     int dlds;
     const int* bf_atom;
     int nat;

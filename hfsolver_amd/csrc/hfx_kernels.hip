@@ -174,7 +174,7 @@ void jk_row_kernel(RowArgs a) {
     __syncthreads();
     for (int k = threadIdx.x; k < nr * nrow * ncols && ndl > 0; k += blockDim.x) {
         const int ri = k / ncols, col = k - ri * ncols;
-        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[col - a.offd] : a.bf_c[col];
+        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[a.col0_d + col - a.offd] : a.bf_c[a.col0_c + col];
         Dl[k] = R(a.D[(size_t)rbf_s[ri] * n + cbf]);
     }
     __syncthreads();
@@ -236,7 +236,9 @@ void jk_row_kernel(RowArgs a) {
 #pragma unroll
                     for (int d = 0; d < ND; d++) Dcd[c][d] = R(kt.Dcd[(size_t)((sc + c) * ND + d) * nk + col]);
             }
-            const int cc = (ki.cp & 0x7fff) + sc, cd = a.offd + ((ki.cp >> 15) & 0x7fff);
+            // LDS columns: positions among the L_C (L_D) functions, less the window's first
+            const int cc = (ki.cp & 0x7fff) - a.col0_c + sc;
+            const int cd = a.offd + ((ki.cp >> 15) & 0x7fff) - (a.offd > 0 ? a.col0_d : a.col0_c);
             const double sket = (ki.cp >> 30) & 1 ? 0.5 : 1.0;
             const KetPrimsSoA<R> kp{sprim, col - soff, npk, ket_pitch(nks)};
             R Jcd[NCt][ND];
@@ -383,7 +385,7 @@ void jk_row_kernel(RowArgs a) {
         const double v = Kl[k];
         if (v == 0.0) continue;
         const int ri = k / ncols, col = k - ri * ncols;
-        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[col - a.offd] : a.bf_c[col];
+        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[a.col0_d + col - a.offd] : a.bf_c[a.col0_c + col];
         unsafeAtomicAdd(&a.K[(size_t)rbf_s[ri] * n + cbf], v);
     }
     for (int k = threadIdx.x; k < nr * NAt * NB; k += blockDim.x) {
@@ -430,7 +432,7 @@ void grad_row_kernel(GradRowArgs a) {
     __syncthreads();
     for (int k = threadIdx.x; k < nr * nrow * ncols && ndl > 0; k += blockDim.x) {
         const int ri = k / ncols, col = k - ri * ncols;
-        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[col - a.offd] : a.bf_c[col];
+        const int cbf = (a.offd > 0 && col >= a.offd) ? a.bf_d[a.col0_d + col - a.offd] : a.bf_c[a.col0_c + col];
         Dl[k] = R(a.D[(size_t)rbf_s[ri] * n + cbf]);
     }
     __syncthreads();
@@ -467,7 +469,9 @@ void grad_row_kernel(GradRowArgs a) {
                 atc = a.bf_atom[ki.fa];
                 atd = a.bf_atom[ki.fb];
             }
-            const int cc = (ki.cp & 0x7fff) + sc, cd = a.offd + ((ki.cp >> 15) & 0x7fff);
+            // LDS columns: positions among the L_C (L_D) functions, less the window's first
+            const int cc = (ki.cp & 0x7fff) - a.col0_c + sc;
+            const int cd = a.offd + ((ki.cp >> 15) & 0x7fff) - (a.offd > 0 ? a.col0_d : a.col0_c);
             const double sket = (ki.cp >> 30) & 1 ? 0.5 : 1.0;
             const KetPrimsSoA<double> kp{sprim, col - soff, npk, ket_pitch(nks)};
             double gcd[6] = {0, 0, 0, 0, 0, 0};   // C, D components summed over the rows

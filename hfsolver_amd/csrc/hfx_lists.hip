@@ -4,8 +4,9 @@
 // schwarz_kernel) this part builds, per geometry:
 //   - the significant pairs (Q_ab Q_max >= threshold) of every pair class,
 //     Q-descending (stable radix sorts: Q, then the class), and the same pairs
-//     grouped into sublists by primitive-pair count bucket (a third, stable
-//     sort on class and bucket), with run lengths of the (class, bucket) keys;
+//     grouped into column windows and, within those, sublists by primitive-pair
+//     count bucket (a third, stable sort on class, window and bucket), with run
+//     lengths of the (class, window, bucket) keys;
 //   - every class's ket table (ket_fill_kernel: records, C - D planes, chunked
 //     primitive planes in FP64 and FP32, and per ket its bound, class position
 //     and primitive count), the per-sublist prefix sums of primitive counts;
@@ -62,15 +63,21 @@ __global__ void pair_key_kernel(const ShellPair* __restrict__ pairs, int np, con
     idx[i] = i;
 }
 
-// Class key (order 1) and class-bucket key (order 2) of the pairs in a given order.
+// Class key (order 1) and class-window-bucket key (order 2: class << 24 |
+// column window << 16 | bucket) of the pairs in a given order; pwin == nullptr:
+// window 0 for every pair.
 __global__ void gather_key_kernel(const ShellPair* __restrict__ pairs, int np, const int* __restrict__ order,
-                                  const unsigned* __restrict__ kc, unsigned* kc_out, unsigned* kb_out) {
+                                  const unsigned* __restrict__ kc, const int* __restrict__ pwin, unsigned* kc_out,
+                                  unsigned* kb_out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= np) return;
     const int pid = order[j];
     const unsigned c = kc[pid];
     if (kc_out) kc_out[j] = c;
-    if (kb_out) kb_out[j] = c == 7u ? (7u << 16) : ((c << 16) | (unsigned)np_bucket_dev(pairs[pid].np));
+    if (kb_out) {
+        const unsigned w = pwin ? (unsigned)pwin[pid] : 0u;
+        kb_out[j] = c == 7u ? (7u << 24) : ((c << 24) | (w << 16) | (unsigned)np_bucket_dev(pairs[pid].np));
+    }
 }
 
 // Position of every significant pair in its class list (order 1, class c at [off[c], off[c+1])).
@@ -290,7 +297,8 @@ int nblocks(long long n, int t = 256) { return (int)((n + t - 1) / t); }
 
 }  // namespace
 
-hipError_t hfx_pair_order(const ShellPair* d_pairs, int np, double thr, PairOrder& o, hipStream_t s) {
+hipError_t hfx_pair_order(const ShellPair* d_pairs, const int* d_pwin, int np, double thr, PairOrder& o,
+                          hipStream_t s) {
     hipError_t e;
 #define HFX_LCHK(x)                    \
     if ((e = (x)) != hipSuccess) {     \
@@ -332,16 +340,18 @@ hipError_t hfx_pair_order(const ShellPair* d_pairs, int np, double thr, PairOrde
     HFX_LCHK(temp_call([&](void* t, size_t& b) {
         return hipcub::DeviceRadixSort::SortPairs(t, b, kq, kq2, idx, idx2, np, 0, 64, s);
     }, s));
-    hipLaunchKernelGGL(gather_key_kernel, dim3(nblocks(np)), dim3(256), 0, s, d_pairs, np, idx2, kc, kc2, nullptr);
+    hipLaunchKernelGGL(gather_key_kernel, dim3(nblocks(np)), dim3(256), 0, s, d_pairs, np, idx2, kc, d_pwin, kc2,
+                       nullptr);
     HFX_LCHK(hipGetLastError());
     HFX_LCHK(temp_call([&](void* t, size_t& b) {
         return hipcub::DeviceRadixSort::SortPairs(t, b, kc2, kc3, idx2, o.cls_order, np, 0, 3, s);
     }, s));
-    // order 2: class and primitive-count bucket (stable in order 1)
-    hipLaunchKernelGGL(gather_key_kernel, dim3(nblocks(np)), dim3(256), 0, s, d_pairs, np, o.cls_order, kc, nullptr, kb);
+    // order 2: class, column window and primitive-count bucket (stable in order 1)
+    hipLaunchKernelGGL(gather_key_kernel, dim3(nblocks(np)), dim3(256), 0, s, d_pairs, np, o.cls_order, kc, d_pwin,
+                       nullptr, kb);
     HFX_LCHK(hipGetLastError());
     HFX_LCHK(temp_call([&](void* t, size_t& b) {
-        return hipcub::DeviceRadixSort::SortPairs(t, b, kb, kb2, o.cls_order, o.sub_order, np, 0, 19, s);
+        return hipcub::DeviceRadixSort::SortPairs(t, b, kb, kb2, o.cls_order, o.sub_order, np, 0, 27, s);
     }, s));
     HFX_LCHK(temp_call([&](void* t, size_t& b) {
         return hipcub::DeviceRunLengthEncode::Encode(t, b, kb2, runkey, runcnt, nruns, np, s);
@@ -357,9 +367,9 @@ hipError_t hfx_pair_order(const ShellPair* d_pairs, int np, double thr, PairOrde
     o.runs.clear();
     for (int c = 0; c < 8; c++) o.ncls[c] = 0;
     for (int r = 0; r < nr; r++) {
-        const int c = (int)(hk[r] >> 16), b = (int)(hk[r] & 0xffff);
+        const int c = (int)(hk[r] >> 24), w = (int)((hk[r] >> 16) & 0xff), b = (int)(hk[r] & 0xffff);
         if (c >= HFX_NPAIRCLASS) continue;
-        o.runs.push_back({c, b, hc[r]});
+        o.runs.push_back({c, w, b, hc[r]});
         o.ncls[c] += hc[r];
     }
     int off[8] = {0};

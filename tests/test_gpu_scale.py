@@ -1,12 +1,13 @@
 """128 waters (n = 3200), twice the benchmark's cluster. The K rows of the
-largest class combinations no longer fit in LDS there: by default those run as
-narrow row launches (sliced over their a-components), with HFX_ROW_NARROW=0 as
-tiles (jk_kernel), the rest in row mode either way. Both builds must equal the
-all-tile build (kernel mode 1) to the atomics' rounding, with the same quartet
-count, and their two-electron gradients (row-mode derivative launches on the
-narrow combinations, the fallback kernel on the tiles) must agree; the
-mixed-precision build must stay within |dE| < 1e-6 Eh of FP64.
-Density: the core-Hamiltonian guess."""
+widest class combinations no longer fit in LDS there. By default the ket
+tables of the widest pair classes are cut into column windows (columns in
+spatial order, one row launch per window); with HFX_ROW_WINDOW_COLS=0 the
+combinations whose rows do not fit run as narrow row launches (sliced over
+their a-components), and with HFX_ROW_NARROW=0 as well as tiles (jk_kernel).
+All three builds must equal the all-tile build (kernel mode 1) to the
+atomics' rounding, with the same quartet count; their two-electron gradients
+must agree; the mixed-precision build must stay within |dE| < 1e-6 Eh of
+FP64. Density: the core-Hamiltonian guess."""
 import sys
 import time
 
@@ -36,55 +37,55 @@ def core_density(H, S, nocc):
     return (2.0 * C @ C.T).cpu().numpy()
 
 
-def test_128_waters_row_and_tile_launches_agree(capfd, monkeypatch):
+def test_128_waters_windows_rows_and_tiles_agree(capfd, monkeypatch):
     atno, xyz = water_cluster(128, seed=128)
     b = Basis(atno, xyz)
     assert b.nbf == 3200
     say("128 waters: engines")
-    t = time.time()
-    eng = Engine(b, verbose=True)
-    err = capfd.readouterr().err
-    assert " rows:" in err and "narrow=1" in err and " tiles:" not in err
-    try:
-        S, T, V = eng.one_electron()
-        P = core_density(T + V, S, int(sum(atno)) // 2)
-        G, J, K, st = eng.fock_2e(P, want_jk=True)
-        t = time.time()
-        gn = np.asarray(eng.grad_2e(P))
-        tg_narrow = time.time() - t
-    finally:
-        eng.close()
-    monkeypatch.setenv("HFX_ROW_NARROW", "0")
-    ew = Engine(b, verbose=True)
-    err = capfd.readouterr().err
-    assert " rows:" in err and " tiles:" in err and "narrow=1" not in err
-    monkeypatch.delenv("HFX_ROW_NARROW")
-    try:
-        Gw, Jw, Kw, stw = ew.fock_2e(P, want_jk=True)
-        t = time.time()
-        gw = np.asarray(ew.grad_2e(P))
-        say("  two-electron gradient, rows and tiles: %.2f s" % (time.time() - t))
-    finally:
-        ew.close()
-    dg = np.abs(gn - gw).max()
-    say("  gradient: max|dg| %.2e of max|g| %.2e" % (dg, np.abs(gw).max()))
-    # (printed after the last capture of the engines' verbose output)
-    say("  narrow rows: quartets %.4g, J/K %.3f s, two-electron gradient %.2f s" % (st.quartets_fp64, st.jk_seconds,
-                                                                                   tg_narrow))
-    assert dg <= 1e-10 * np.abs(gw).max()
-    say("  rows and tiles: quartets %.4g, J/K %.3f s" % (stw.quartets_fp64, stw.jk_seconds))
+    variants = [("column windows", {}, lambda e: "window 1/2" in e and " tiles:" not in e),
+                ("narrow rows", {"HFX_ROW_WINDOW_COLS": "0"}, lambda e: "narrow=1" in e and " tiles:" not in e),
+                ("rows and tiles", {"HFX_ROW_WINDOW_COLS": "0", "HFX_ROW_NARROW": "0"},
+                 lambda e: " rows:" in e and " tiles:" in e and "narrow=1" not in e)]
+    out, lines, P = {}, [], None
+    for name, env, check in variants:
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        eng = Engine(b, verbose=True)
+        for k in env:
+            monkeypatch.delenv(k)
+        err = capfd.readouterr().err
+        assert check(err), name
+        try:
+            if P is None:
+                S, T, V = eng.one_electron()
+                P = core_density(T + V, S, int(sum(atno)) // 2)
+            G, J, K, st = eng.fock_2e(P, want_jk=True)
+            t = time.time()
+            g = np.asarray(eng.grad_2e(P))
+            tg = time.time() - t
+        finally:
+            eng.close()
+        out[name] = (G, J, K, g, st)
+        lines.append("  %s: quartets %.4g, J/K %.3f s, two-electron gradient %.2f s" % (name, st.quartets_fp64,
+                                                                                       st.jk_seconds, tg))
     et = Engine(b, kernel_mode=1)
     try:
         Gt, Jt, Kt, stt = et.fock_2e(P, want_jk=True)
     finally:
         et.close()
+    for line in lines:   # after the last capture of the engines' verbose output
+        say(line)
     say("  all tiles: quartets %.4g, J/K %.3f s" % (stt.quartets_fp64, stt.jk_seconds))
-    assert stt.quartets_fp64 == st.quartets_fp64 == stw.quartets_fp64
-    for name, (Jx, Kx) in (("narrow rows", (J, K)), ("rows and tiles", (Jw, Kw))):
-        dJ, dK = np.abs(Jx - Jt).max(), np.abs(Kx - Kt).max()
-        say("  %s vs all tiles: max|dJ| %.2e max|dK| %.2e" % (name, dJ, dK))
+    gref = out["rows and tiles"][3]
+    for name, (G, J, K, g, st) in out.items():
+        assert st.quartets_fp64 == stt.quartets_fp64
+        dJ, dK = np.abs(J - Jt).max(), np.abs(K - Kt).max()
+        dg = np.abs(g - gref).max()
+        say("  %s vs all tiles: max|dJ| %.2e max|dK| %.2e; gradient max|dg| %.2e" % (name, dJ, dK, dg))
         assert dJ <= 1e-12 * np.abs(Jt).max() and dK <= 1e-12 * np.abs(Kt).max()
-    assert np.abs(G - G.T).max() < 1e-12
+        assert dg <= 1e-10 * np.abs(gref).max()
+        assert np.abs(G - G.T).max() < 1e-12
+    G = out["column windows"][0]
     em = Engine(b, fp32_threshold=1e-5)
     try:
         _, nq64, nq32 = em.screening()
