@@ -342,15 +342,29 @@ int build_shells(hfx_t* h, int nbf, const int* nprim, const int* istart, const d
         i += nc;
     }
     h->nbf = nbf;
-    // column windows per pair class, from the widths of its ket rows
-    bool windows = false;
+    // column windows per pair class: only for a ket class whose full K rows (no
+    // density staging) would not fit in LDS for some bra class of the basis, in
+    // ranges of at most about win_cols columns. (Where the rows fit, windows
+    // cost more than they save: the D columns a window reaches extend the pair
+    // range past its C columns, and the blocks multiply.)
+    bool windows = false, haveL[HFX_LMAX + 1] = {};
+    for (const Shell& sh : h->shells) haveL[sh.L] = true;
     for (int c = 0; c < HFX_NPAIRCLASS; c++) {
         const int lc = pair_class_la(c), ld = pair_class_lb(c);
         int ncols = 0;
         for (const Shell& sh : h->shells)
             if (sh.L == lc || (sh.L == ld && ld != lc)) ncols += ncart(sh.L);
-        int W = h->win_cols > 0 ? (ncols + h->win_cols - 1) / h->win_cols : 1;
-        h->nwin[c] = std::max(1, std::min(W, HFX_WIN_MAX));
+        bool wide = false;
+        for (int cb = c; cb < HFX_NPAIRCLASS && h->win_cols > 0; cb++) {
+            if (!haveL[pair_class_la(cb)] || !haveL[pair_class_lb(cb)]) continue;
+            for (int prec = 0; prec < 2; prec++) {
+                int nrow = 0, nab = 0;
+                hfx_row_shape(cb, c, prec, &nrow, &nab);
+                wide = wide || hfx_row_lds(nrow, nab, ncols, 1, 0, prec == 1) > h->lds_budget;
+            }
+        }
+        const int W = wide ? std::max(2, (ncols + h->win_cols - 1) / h->win_cols) : 1;
+        h->nwin[c] = std::min(W, HFX_WIN_MAX);
         windows = windows || h->nwin[c] > 1;
     }
     // LDS columns: the functions of the shells of each L, in shell order, or
