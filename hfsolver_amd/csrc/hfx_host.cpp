@@ -139,6 +139,7 @@ struct hfx_engine {
         int col0_c = 0, col0_d = 0;   // the window's first LDS columns (RowArgs)
         int R = 1, dlds = 0, ncols = 0, offd = 0;   // dlds: bit 0 FP64 launch, bit 1 FP32 launch
         bool narrow = false;   // launches sliced by split_a_rows / split_c_rows (K rows too wide otherwise)
+        bool jk = true, grad = true;   // used by the J/K launches / by the derivative launches
         int nblk[2] = {0, 0};
         int chunk[2] = {0, 0};
         double cost[2] = {0, 0};   // modelled flops of the launch (orders the launches)
@@ -364,7 +365,7 @@ int build_shells(hfx_t* h, int nbf, const int* nprim, const int* istart, const d
             }
         }
         const int W = wide ? std::max(2, (ncols + h->win_cols - 1) / h->win_cols) : 1;
-        h->nwin[c] = std::min(W, HFX_WIN_MAX);
+        h->nwin[c] = std::min(W, HFX_WIN_MAX - 1);   // slot nwin: the class's full table
         windows = windows || h->nwin[c] > 1;
     }
     // LDS columns: the functions of the shells of each L, in shell order, or
@@ -692,8 +693,8 @@ int build_lists(hfx_t* h) {
         }
         for (int c = 0; c < HFX_NPAIRCLASS; c++) {
             const int lc = pair_class_la(c), ld = pair_class_lb(c);
-            for (int w = 0; w < h->nwin[c]; w++) {
-                if (h->nwin[c] == 1) {
+            for (int w = 0; w <= h->nwin[c] && w < HFX_WIN_MAX; w++) {
+                if (h->nwin[c] == 1 || w == h->nwin[c]) {   // all columns
                     h->win_c0[c][w] = 0;
                     h->win_cn[c][w] = (int)h->bfL[lc].size();
                     h->win_d0[c][w] = 0;
@@ -714,8 +715,16 @@ int build_lists(hfx_t* h) {
     if (dmalloc(h, (void**)&po.pos, (size_t)np * sizeof(int), true)) return 1;
     HIPCHK(h, hfx_pair_order(h->d_pairs, d_pwin, np, thr, po, h->stream));
     h->npairs_sig = po.nsig;
+    // with windows also the (class, bucket) order of the full tables
+    PairOrder pof;
+    if (d_pwin) {
+        if (dmalloc(h, (void**)&pof.cls_order, (size_t)np * sizeof(int), true)) return 1;
+        if (dmalloc(h, (void**)&pof.sub_order, (size_t)np * sizeof(int), true)) return 1;
+        if (dmalloc(h, (void**)&pof.pos, (size_t)np * sizeof(int), true)) return 1;
+        HIPCHK(h, hfx_pair_order(h->d_pairs, nullptr, np, thr, pof, h->stream));
+    }
     // class lists: views of the sorted array on the device, a host copy for the counts and the tile lists
-    std::vector<int> order(po.nsig), hsub;
+    std::vector<int> order(po.nsig), hsub, hsubF;
     if (po.nsig > 0)
         HIPCHK(h, hipMemcpy(order.data(), po.cls_order, (size_t)po.nsig * sizeof(int), hipMemcpyDeviceToHost));
     int cls_off[HFX_NPAIRCLASS + 1] = {0};
@@ -734,16 +743,27 @@ int build_lists(hfx_t* h) {
             subs[r[0]][r[1]].push_back({r[2], r[3]});
             off += r[3];
         }
+        // full tables of the windowed classes in slot nwin (offsets into pof)
+        off = 0;
+        for (const auto& r : pof.runs) {
+            const int c = r[0], W = h->nwin[c];
+            if (W > 1) {
+                if (subs[c][W].empty()) sub0[c][W] = off;
+                subs[c][W].push_back({r[2], r[3]});
+            }
+            off += r[3];
+        }
     }
     KetAux aux[HFX_NPAIRCLASS][HFX_WIN_MAX] = {};
     const auto tk = std::chrono::steady_clock::now();
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        for (int w = 0; w < h->nwin[c]; w++) {
+        for (int w = 0; w <= h->nwin[c] && w < HFX_WIN_MAX; w++) {
             if (subs[c][w].empty()) continue;
             if ((int)subs[c][w].size() > HFX_SUB_MAX)
                 return fail(h, "pair class %d has %zu primitive-count sublists (at most %d)", c, subs[c][w].size(),
                             HFX_SUB_MAX);
-            if (build_ket_table(h, c, w, subs[c][w], sub0[c][w], po, aux[c][w])) return 1;
+            const bool full = h->nwin[c] > 1 && w == h->nwin[c];
+            if (build_ket_table(h, c, w, subs[c][w], sub0[c][w], full ? pof : po, aux[c][w])) return 1;
         }
     if (h->opts.verbose) {
         HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -753,6 +773,7 @@ int build_lists(hfx_t* h) {
     struct Pending {
         int cb, ck, w, nb, S, R, dlds;
         bool rowmode;
+        bool gonly;           // blocks for the derivative launches only (no J/K, no statistics)
         size_t stat;          // first entry of its statistics
         int* rkb[2];
         int* rke[2];
@@ -760,78 +781,115 @@ int build_lists(hfx_t* h) {
     };
     std::vector<Pending> pend;
     size_t nstat = 0;
+    // one pending launch set per ket-table slot: slot w < nwin of a windowed
+    // class, slot nwin its full table, slot 0 the table of a class without windows
+    auto plan = [&](int cb, int ck, int w, Pending& q) {
+        q = Pending{};
+        q.cb = cb;
+        q.ck = ck;
+        q.w = w;
+        q.nb = po.ncls[cb];
+        q.S = (int)subs[ck][w].size();
+        hfx_engine::RowCombo& rc = q.rc;
+        rc.cb = cb;
+        rc.ck = ck;
+        rc.win = w;
+        const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
+        // LDS columns: the window's (all columns without windows)
+        rc.ncols = h->win_cn[ck][w] + (ld != lc ? h->win_dn[ck][w] : 0);
+        rc.offd = ld != lc ? h->win_cn[ck][w] : 0;
+        rc.col0_c = h->win_c0[ck][w];
+        rc.col0_d = ld != lc ? h->win_d0[ck][w] : h->win_c0[ck][w];
+        // the FP64 and FP32 launches slice differently and decide the staging of
+        // the density rows each within its own budget (dlds bit 0: the FP64
+        // launch, bit 1: the FP32 launch); the blocks are planned for the
+        // larger LDS of the two
+        int nrow64 = 0, nab64 = 0, nrow32 = 0, nab32 = 0;
+        hfx_row_shape(cb, ck, 0, &nrow64, &nab64);
+        hfx_row_shape(cb, ck, 1, &nrow32, &nab32);
+        auto staging = [&]() {
+            int m = 0;
+            if (h->row_dlds) {
+                if (hfx_row_lds(nrow64, nab64, rc.ncols, 1, 1, false) <= std::min(h->lds_budget, h->dlds_budget))
+                    m |= 1;
+                if (hfx_row_lds(nrow32, nab32, rc.ncols, 1, 1, true) <= std::min(h->lds_budget, h->dlds_budget32))
+                    m |= 2;
+            }
+            return m;
+        };
+        auto row_lds = [&](int R, int m) {
+            return std::max(hfx_row_lds(nrow64, nab64, rc.ncols, R, m & 1, false),
+                            hfx_row_lds(nrow32, nab32, rc.ncols, R, (m >> 1) & 1, true));
+        };
+        int dlds = staging();
+        const size_t need = row_lds(1, dlds);
+        // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
+        int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
+        if (R == 0 && need <= h->lds_budget) R = 1;
+        if (R == 0 && h->row_narrow) {
+            // narrow launches: one K row of the a-shell per bra, one row per block
+            hfx_row_shape(cb, ck, 4, &nrow64, &nab64);
+            hfx_row_shape(cb, ck, 5, &nrow32, &nab32);
+            dlds = staging();
+            if (row_lds(1, dlds) <= h->lds_budget) {
+                R = 1;
+                rc.narrow = true;
+            }
+        }
+        q.rowmode = h->opts.kernel_mode == 0 && R >= 1;
+        q.R = R;
+        q.dlds = dlds;
+        rc.R = R;
+        rc.dlds = dlds;
+    };
+    auto push = [&](Pending& q) -> int {
+        for (int prec = 0; prec < 2; prec++) {
+            if (dmalloc(h, (void**)&q.rkb[prec], (size_t)q.nb * q.S * sizeof(int), true)) return 1;
+            if (dmalloc(h, (void**)&q.rke[prec], (size_t)q.nb * q.S * sizeof(int), true)) return 1;
+            q.rc.d_kbeg[prec] = q.rkb[prec];
+            q.rc.d_kend[prec] = q.rke[prec];
+        }
+        q.stat = nstat;
+        nstat += (size_t)q.S * 4;
+        pend.push_back(q);
+        return 0;
+    };
+    for (int c = 0; c < HFX_NPAIRCLASS; c++)
+        if ((int)h->bfL[pair_class_la(c)].size() >= 32768 || (int)h->bfL[pair_class_lb(c)].size() >= 32768)
+            return fail(h, "ket table: too many functions of one angular momentum");
     for (int cb = 0; cb < HFX_NPAIRCLASS; cb++)
-        for (int ck = 0; ck <= cb; ck++)
-            for (int w = 0; w < h->nwin[ck]; w++) {
-            if (po.ncls[cb] == 0 || subs[ck][w].empty()) continue;
-            Pending q{};
-            q.cb = cb;
-            q.ck = ck;
-            q.w = w;
-            q.nb = po.ncls[cb];
-            q.S = (int)subs[ck][w].size();
-            hfx_engine::RowCombo& rc = q.rc;
-            rc.cb = cb;
-            rc.ck = ck;
-            rc.win = w;
-            const int lc = pair_class_la(ck), ld = pair_class_lb(ck);
-            if ((int)h->bfL[lc].size() >= 32768 || (int)h->bfL[ld].size() >= 32768)
-                return fail(h, "ket table: too many functions of one angular momentum");
-            // LDS columns: the window's (all columns without windows)
-            rc.ncols = h->win_cn[ck][w] + (ld != lc ? h->win_dn[ck][w] : 0);
-            rc.offd = ld != lc ? h->win_cn[ck][w] : 0;
-            rc.col0_c = h->win_c0[ck][w];
-            rc.col0_d = ld != lc ? h->win_d0[ck][w] : h->win_c0[ck][w];
-            // the FP64 and FP32 launches slice differently and decide the staging of
-            // the density rows each within its own budget (dlds bit 0: the FP64
-            // launch, bit 1: the FP32 launch); the blocks are planned for the
-            // larger LDS of the two
-            int nrow64 = 0, nab64 = 0, nrow32 = 0, nab32 = 0;
-            hfx_row_shape(cb, ck, 0, &nrow64, &nab64);
-            hfx_row_shape(cb, ck, 1, &nrow32, &nab32);
-            auto staging = [&]() {
-                int m = 0;
-                if (h->row_dlds) {
-                    if (hfx_row_lds(nrow64, nab64, rc.ncols, 1, 1, false) <= std::min(h->lds_budget, h->dlds_budget))
-                        m |= 1;
-                    if (hfx_row_lds(nrow32, nab32, rc.ncols, 1, 1, true) <= std::min(h->lds_budget, h->dlds_budget32))
-                        m |= 2;
-                }
-                return m;
-            };
-            auto row_lds = [&](int R, int m) {
-                return std::max(hfx_row_lds(nrow64, nab64, rc.ncols, R, m & 1, false),
-                                hfx_row_lds(nrow32, nab32, rc.ncols, R, (m >> 1) & 1, true));
-            };
-            int dlds = staging();
-            const size_t need = row_lds(1, dlds);
-            // rows per block: as many (up to row_rmax) as stay within row_pair_bytes of LDS
-            int R = (int)std::min<size_t>(h->row_rmax, h->row_pair_bytes / need);
-            if (R == 0 && need <= h->lds_budget) R = 1;
-            if (R == 0 && h->row_narrow) {
-                // narrow launches: one K row of the a-shell per bra, one row per block
-                hfx_row_shape(cb, ck, 4, &nrow64, &nab64);
-                hfx_row_shape(cb, ck, 5, &nrow32, &nab32);
-                dlds = staging();
-                if (row_lds(1, dlds) <= h->lds_budget) {
-                    R = 1;
-                    rc.narrow = true;
-                }
+        for (int ck = 0; ck <= cb; ck++) {
+            if (po.ncls[cb] == 0 || po.ncls[ck] == 0) continue;
+            const int W = h->nwin[ck];
+            Pending q;
+            if (W == 1) {
+                plan(cb, ck, 0, q);
+                if (push(q)) return 1;
+                continue;
             }
-            q.rowmode = h->opts.kernel_mode == 0 && R >= 1;
-            q.R = R;
-            q.dlds = dlds;
-            rc.R = R;
-            rc.dlds = dlds;
-            for (int prec = 0; prec < 2; prec++) {
-                if (dmalloc(h, (void**)&q.rkb[prec], (size_t)q.nb * q.S * sizeof(int), true)) return 1;
-                if (dmalloc(h, (void**)&q.rke[prec], (size_t)q.nb * q.S * sizeof(int), true)) return 1;
-                rc.d_kbeg[prec] = q.rkb[prec];
-                rc.d_kend[prec] = q.rke[prec];
+            // windowed class: the windows for J/K (measured faster than the full
+            // table for every combination of a class whose rows outgrow the LDS
+            // somewhere) and the full table for the derivative launches, which
+            // hold no K rows; tiles (kernel mode 1) over the full table
+            plan(cb, ck, W, q);
+            if (h->opts.kernel_mode == 1) {
+                if (push(q)) return 1;
+                continue;
             }
-            q.stat = nstat;
-            nstat += (size_t)q.S * 4;
-            pend.push_back(q);
+            for (int w = 0; w < W; w++) {
+                if (subs[ck][w].empty()) continue;
+                Pending qw;
+                plan(cb, ck, w, qw);
+                qw.rc.grad = false;
+                if (push(qw)) return 1;
+            }
+            q.rowmode = true;   // gradient only: row blocks over the full table (no K rows)
+            q.gonly = true;
+            q.R = q.rc.R = 1;
+            q.dlds = q.rc.dlds = 0;
+            q.rc.narrow = false;
+            q.rc.jk = false;
+            if (push(q)) return 1;
         }
     std::vector<double> stats(nstat), sums(pend.size() * 4, 0.0);
     DeviceTemp d_stats, d_sums;
@@ -871,6 +929,12 @@ int build_lists(hfx_t* h) {
         hsub.resize(po.nsig);
         if (po.nsig > 0)
             HIPCHK(h, hipMemcpy(hsub.data(), po.sub_order, (size_t)po.nsig * sizeof(int), hipMemcpyDeviceToHost));
+        if (d_pwin) {
+            hsubF.resize(pof.nsig);
+            if (pof.nsig > 0)
+                HIPCHK(h, hipMemcpy(hsubF.data(), pof.sub_order, (size_t)pof.nsig * sizeof(int),
+                                    hipMemcpyDeviceToHost));
+        }
     }
     for (size_t i = 0; i < pend.size(); i++) {
         Pending& q = pend[i];
@@ -883,7 +947,8 @@ int build_lists(hfx_t* h) {
                 HIPCHK(h, hipMemcpy(hkb[prec].data(), q.rkb[prec], hkb[prec].size() * sizeof(int), hipMemcpyDeviceToHost));
                 HIPCHK(h, hipMemcpy(hke[prec].data(), q.rke[prec], hke[prec].size() * sizeof(int), hipMemcpyDeviceToHost));
             }
-        for (int s = 0; s < S; s++) {
+        // the launches' statistics and tile lists (none for the derivative-only blocks)
+        for (int s = 0; s < S && !q.gonly; s++) {
             hfx_engine::Combo cmb;
             cmb.cb = cb;
             cmb.ck = ck;
@@ -901,7 +966,8 @@ int build_lists(hfx_t* h) {
             } else {
                 // tile mode: the sublist's ket list, per-row ranges relative to it, tiles, gradient blocks
                 const int soff = h->ktab[ck][w].t.off[s];
-                std::vector<int> kl(hsub.begin() + sub0[ck][w] + soff, hsub.begin() + sub0[ck][w] + soff + cmb.nket);
+                const std::vector<int>& hs = (h->nwin[ck] > 1 && w == h->nwin[ck]) ? hsubF : hsub;
+                std::vector<int> kl(hs.begin() + sub0[ck][w] + soff, hs.begin() + sub0[ck][w] + soff + cmb.nket);
                 if (upload(h, &cmb.d_ket, kl, true)) return 1;
                 for (int prec = 0; prec < 2; prec++) {
                     std::vector<int> kb(nb), ke(nb), ti0, tj0;
@@ -990,9 +1056,10 @@ int build_lists(hfx_t* h) {
                     c.npk, c.rowmode ? "rows" : "tiles", c.nq[0], c.nq[1], c.nprimq[0], c.nprimq[1]);
         for (auto& c : h->rcombos)
             fprintf(stderr,
-                    "hfx:   rows (%d%d|%d%d) window %d/%d: R=%d dlds=%d narrow=%d ncols=%d blocks %d / %d chunk %d / %d\n",
+                    "hfx:   rows (%d%d|%d%d) window %d/%d%s: R=%d dlds=%d narrow=%d ncols=%d blocks %d / %d chunk %d / %d\n",
                     pair_class_la(c.cb), pair_class_lb(c.cb), pair_class_la(c.ck), pair_class_lb(c.ck), c.win,
-                    h->nwin[c.ck], c.R, c.dlds, (int)c.narrow, c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
+                    h->nwin[c.ck], c.jk ? (c.grad ? "" : " (J/K)") : " (gradient)", c.R, c.dlds, (int)c.narrow,
+                    c.ncols, c.nblk[0], c.nblk[1], c.chunk[0], c.chunk[1]);
     }
     return 0;
 }
@@ -1188,7 +1255,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     if (h->dthr > 0)
         HIPCHK(h, hfx_launch_shell_dmax(h->d_D, n, h->d_dsh_f0, h->d_dsh_nc, nsh, h->d_dsh, h->stream));
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        for (int w = 0; w < h->nwin[c]; w++)
+        for (int w = 0; w < HFX_WIN_MAX; w++)   // windows and full tables
             if (h->ktab[c][w].t.nk > 0)
                 HIPCHK(h, hfx_launch_ket_gather(h->ktab[c][w].t, h->ktab[c][w].ncomp, h->ktab[c][w].nd, h->d_D, n,
                                                 h->stream));
@@ -1204,7 +1271,8 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
     std::vector<std::pair<double, int>> order;
     for (size_t i = 0; i < h->rcombos.size(); i++)
         for (int prec = 0; prec < 2; prec++)
-            if (h->rcombos[i].nblk[prec] > 0) order.push_back({h->rcombos[i].cost[prec], (int)(2 * i + prec)});
+            if (h->rcombos[i].jk && h->rcombos[i].nblk[prec] > 0)
+                order.push_back({h->rcombos[i].cost[prec], (int)(2 * i + prec)});
     std::stable_sort(order.begin(), order.end(), [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
         return x.first > y.first;
     });
@@ -1295,7 +1363,7 @@ int hfx_fock_2e_device(hfx_t* h, const double* dP, double* dG, double* dJ, doubl
         }
     }
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        for (int w = 0; w < h->nwin[c]; w++)
+        for (int w = 0; w < HFX_WIN_MAX; w++)   // windows and full tables
             if (h->ktab[c][w].t.nk > 0)
                 HIPCHK(h, hfx_launch_ket_scatter(h->ktab[c][w].t, h->ktab[c][w].ncomp, h->ktab[c][w].nd, h->d_Jb, n,
                                                  h->stream));
@@ -1417,7 +1485,7 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     // row mode over the ket tables where the J/K launch is in row mode
     for (int c = 0; c < HFX_NPAIRCLASS; c++)
-        for (int w = 0; w < h->nwin[c]; w++)
+        for (int w = 0; w < HFX_WIN_MAX; w++)   // windows and full tables
             if (h->ktab[c][w].t.nk > 0)
                 HIPCHK(h, hfx_launch_ket_gather(h->ktab[c][w].t, h->ktab[c][w].ncomp, h->ktab[c][w].nd, h->d_D, n,
                                                 h->stream));
@@ -1431,7 +1499,8 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     std::vector<std::pair<double, int>> order;
     for (size_t i = 0; i < h->rcombos.size(); i++)
         for (int prec = 0; prec < 2; prec++)
-            if (h->rcombos[i].nblk[prec] > 0) order.push_back({h->rcombos[i].cost[prec], (int)(2 * i + prec)});
+            if (h->rcombos[i].grad && h->rcombos[i].nblk[prec] > 0)
+                order.push_back({h->rcombos[i].cost[prec], (int)(2 * i + prec)});
     std::stable_sort(order.begin(), order.end(), [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
         return x.first > y.first;
     });
@@ -1493,7 +1562,8 @@ int hfx_grad_2e(hfx_t* h, const double* P, double* g) {
     a.grad = h->d_grad;
     a.n = n;
     for (const auto& c : h->combos) {
-        if (done[c.cb][c.ck][c.win]) continue;
+        // the derivative launches of a windowed class ran over its full table
+        if (done[c.cb][c.ck][c.win] || (h->nwin[c.ck] > 1 && done[c.cb][c.ck][h->nwin[c.ck]])) continue;
         if (3 * (size_t)h->nat * sizeof(double) > HFX_GRAD_LDS_MAX)
             return fail(h, "hfx_grad_2e: %d atoms need %zu bytes of LDS in the fallback gradient kernel (at most %zu)",
                         h->nat, 3 * (size_t)h->nat * sizeof(double), HFX_GRAD_LDS_MAX);
