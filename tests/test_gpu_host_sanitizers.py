@@ -10,7 +10,8 @@ nuclei, explicit quartets with and without allocation, store mode, and the
 rejected inputs (NaN in P, a short int2 array, inconsistent basis arrays).
 Any invalid host access, leak of the engine's own allocations or undefined
 behaviour ends it with a report; its G and gradient must equal the normal
-library's.
+library's. At 128 waters the (10) kets run in column windows, with their
+full table for the gradient, and (22|10) in narrow rows.
 """
 import os
 import subprocess
@@ -34,7 +35,7 @@ def write_basis(path, b, P):
         np.asfortranarray(P, dtype=np.float64).ravel(order="F").tofile(f)
 
 
-@pytest.mark.parametrize("nw", [2, 4])
+@pytest.mark.parametrize("nw", [2, 4, 128])
 def test_host_runtime_under_asan_and_ubsan(nw, tmp_path):
     from hfsolver_amd.basis import Basis
     from hfsolver_amd.clusters import water_cluster
@@ -56,7 +57,8 @@ def test_host_runtime_under_asan_and_ubsan(nw, tmp_path):
     r = subprocess.run([KC.HOST_SAN_EXE, bf, of], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-6000:]
     assert "host_san: ok" in r.stdout
-    for what in ("NaN in P", "short array", "istart beyond"):
+    # (the store mode, with its short-array check, runs up to n = 100)
+    for what in ("NaN in P", "istart beyond") + (("short array",) if b.nbf <= 100 else ()):
         assert what in r.stdout
     out = np.fromfile(of, dtype=np.float64)
     n, nat = b.nbf, len(b.atno)
@@ -68,5 +70,6 @@ def test_host_runtime_under_asan_and_ubsan(nw, tmp_path):
         g0 = eng.grad_2e(P)
     finally:
         eng.close()
-    assert np.abs(G - G0).max() < 1e-12
-    assert np.abs(g - g0).max() < 1e-11
+    # the same engine code, summed by atomics in another order
+    assert np.abs(G - G0).max() < 1e-12 * max(1.0, np.abs(G0).max())
+    assert np.abs(g - g0).max() < 1e-11 * max(1.0, np.abs(g0).max())
