@@ -86,9 +86,9 @@ struct hfx_engine {
     // block per CU)
     size_t lds_budget = HFX_ROW_LDS_MAX;
     // largest row-mode LDS per block with the density rows staged, FP64 and FP32
-    // launches (single-precision rows: the FP32 launches gain from staging them
-    // up to the whole budget, the FP64 launches lose above 120 KB)
-    size_t dlds_budget = 120 * 1024;
+    // launches (both the whole row budget; the classes that read the density
+    // through the caches instead: hfx_row_stage_density)
+    size_t dlds_budget = HFX_ROW_LDS_MAX;
     size_t dlds_budget32 = HFX_ROW_LDS_MAX;
     // pairs
     std::vector<ShellPair> pairs;

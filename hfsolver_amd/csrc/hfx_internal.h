@@ -311,11 +311,12 @@ HFX_HD constexpr int hfx_row_threads(int X, int Y, bool fp32) {
 // whose launches measured faster reading the density through the caches
 // (64 waters, launches serialised: FP64 (10|10) -1 % in the FP64 build and
 // -16 % in the mixed build, (11|11) -4 %, (21|21) -10 %, (22|11) -1 %,
-// (22|22) -9 %; FP32 (22|22) -12 %); every other class is faster staging them.
+// (22|22) -9 %, (20|10) -17 % (its rows take 136 KB with the density, one
+// block per CU); FP32 (22|22) -12 %); every other class is faster staging them.
 HFX_HD constexpr bool hfx_row_stage_density(int X, int Y, bool fp32) {
     if (fp32) return !(X == 5 && Y == 5);
     return !((X == 1 && Y == 1) || (X == 2 && Y == 2) || (X == 4 && Y == 4) || (X == 5 && Y == 2) ||
-             (X == 5 && Y == 5));
+             (X == 5 && Y == 5) || (X == 3 && Y == 1));
 }
 // Threads of a gradient row-mode block of class (X|Y) in FP32 or FP64: 256,
 // or 512 for the FP32 classes (21|11), (21|10), (20|11) (10-24 % faster per
